@@ -1,0 +1,177 @@
+"""ctypes binding of libblast_hip.so (the C ABI declared in include/blast_hip.h).
+
+Device buffers are torch tensors on ``cuda:0`` (PyTorch is used for device
+memory and streams only; every computation runs in the HIP kernels of
+libblast_hip.so).  If the library is missing, every call raises: there is no
+CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from .layout import BlastMatrix, Layout
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libblast_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "blast_hip.h")
+
+ALGO_AUTO = 0
+ALGO_REFERENCE = 1
+ERR_RUNTIME = -1000
+
+_lib = None
+
+
+class BlastError(RuntimeError):
+    def __init__(self, func, code, msg):
+        super().__init__(f"{func} returned {code}: {msg}")
+        self.code = code
+
+
+def declared_symbols() -> list[str]:
+    """Every function declared in include/blast_hip.h."""
+    with open(HEADER_PATH) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(blast_hip_\w+)\s*\(", text)))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER(BlastMatrix)
+        vp, i64, i32p, c_int = ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int
+        for p, real in (("d", ctypes.c_double), ("s", ctypes.c_float)):
+            getattr(L, f"blast_hip_{p}gemm_batched").argtypes = [vp, real, P, P, real, P, P, i64]
+            getattr(L, f"blast_hip_{p}syrk_lower_batched").argtypes = [vp, real, P, real, P, P, i64]
+            getattr(L, f"blast_hip_{p}trmm_batched").argtypes = [vp, c_int, c_int, real, P, P, P, i64]
+            getattr(L, f"blast_hip_{p}trsm_batched").argtypes = [vp, c_int, c_int, c_int, real, P, P, P, i64]
+            getattr(L, f"blast_hip_{p}potrf_batched").argtypes = [vp, P, P, i64, i32p]
+            getattr(L, f"blast_hip_{p}getrf_batched").argtypes = [vp, P, vp, i64, i64, i32p]
+        L.blast_hip_dposv_batched.argtypes = [vp, P, P, P, P, i64, i32p]
+        L.blast_hip_set_algorithm.argtypes = [c_int]
+        L.blast_hip_last_error.restype = ctypes.c_char_p
+        L.blast_hip_version.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().blast_hip_last_error().decode()
+
+
+def set_algorithm(algo: int) -> None:
+    r = lib().blast_hip_set_algorithm(algo)
+    if r:
+        raise BlastError("blast_hip_set_algorithm", r, "bad algorithm")
+
+
+def check(name, code):
+    if code != 0:
+        raise BlastError(name, code, last_error())
+    return code
+
+
+def _prefix(dtype):
+    return "d" if np.dtype(dtype) == np.float64 else "s"
+
+
+def _stream(stream):
+    if stream is not None:
+        return stream
+    import torch
+
+    return torch.cuda.current_stream().cuda_stream
+
+
+class Operand:
+    """A batch operand on the device: a torch tensor plus its storage Layout."""
+
+    def __init__(self, layout: Layout, tensor):
+        self.layout = layout
+        self.tensor = tensor
+
+    @staticmethod
+    def from_host(layout: Layout, host_buf: np.ndarray, device="cuda"):
+        import torch
+
+        return Operand(layout, torch.from_numpy(np.ascontiguousarray(host_buf)).to(device))
+
+    @staticmethod
+    def empty(layout: Layout, batch: int, device="cuda", fill=0.0):
+        import torch
+
+        tdt = torch.float64 if layout.dtype == np.float64 else torch.float32
+        return Operand(layout, torch.full((batch * layout.batch_stride,), fill, dtype=tdt, device=device))
+
+    def t(self) -> "Operand":
+        return Operand(self.layout.t(), self.tensor)
+
+    def desc(self) -> BlastMatrix:
+        return self.layout.descriptor(self.tensor.data_ptr())
+
+    def host(self) -> np.ndarray:
+        return self.tensor.cpu().numpy()
+
+    def logical(self, batch) -> np.ndarray:
+        return self.layout.unpack(self.host(), batch)
+
+
+def _ref(op: Operand):
+    return ctypes.byref(op.desc())
+
+
+def gemm(alpha, A, B, beta, C, D, batch, stream=None):
+    p = _prefix(D.layout.dtype)
+    name = f"blast_hip_{p}gemm_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), alpha, _ref(A), _ref(B), beta, _ref(C), _ref(D), batch))
+
+
+def syrk_lower(alpha, A, beta, C, D, batch, stream=None):
+    p = _prefix(D.layout.dtype)
+    name = f"blast_hip_{p}syrk_lower_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), alpha, _ref(A), beta, _ref(C), _ref(D), batch))
+
+
+def trmm(side_right, upper, alpha, A, B, C, batch, stream=None):
+    p = _prefix(C.layout.dtype)
+    name = f"blast_hip_{p}trmm_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), int(side_right), int(upper), alpha, _ref(A), _ref(B),
+                                            _ref(C), batch))
+
+
+def trsm(side_right, upper, unit, alpha, A, B, X, batch, stream=None):
+    p = _prefix(X.layout.dtype)
+    name = f"blast_hip_{p}trsm_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), int(side_right), int(upper), int(unit), alpha, _ref(A),
+                                            _ref(B), _ref(X), batch))
+
+
+def _info_ptr(info):
+    return None if info is None else info.data_ptr()
+
+
+def potrf(A, L, batch, info=None, stream=None):
+    p = _prefix(L.layout.dtype)
+    name = f"blast_hip_{p}potrf_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), _ref(A), _ref(L), batch, _info_ptr(info)))
+
+
+def getrf(A, ipiv, ipiv_stride, batch, info=None, stream=None):
+    p = _prefix(A.layout.dtype)
+    name = f"blast_hip_{p}getrf_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), _ref(A), ipiv.data_ptr() if ipiv is not None else None,
+                                            ipiv_stride, batch, _info_ptr(info)))
+
+
+def posv(A, L, B, X, batch, info=None, stream=None):
+    name = "blast_hip_dposv_batched"
+    return check(name, lib().blast_hip_dposv_batched(_stream(stream), _ref(A), _ref(L), _ref(B), _ref(X), batch,
+                                                     _info_ptr(info)))

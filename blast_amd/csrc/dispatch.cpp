@@ -1,0 +1,59 @@
+// Chooses the kernel for each call.  BLAST_HIP_ALGO_REFERENCE always takes the
+// reference-order kernels; AUTO takes the MFMA / fused kernels where the shape
+// and layout qualify.
+#include "dispatch.hpp"
+
+namespace blast_hip {
+
+template <typename T>
+int dispatch_gemm(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D, int64_t batch) {
+    return launch_gemm_ref<T>(s, alpha, A, B, beta, C, D, batch);
+}
+
+template <typename T>
+int dispatch_syrk(hipStream_t s, T alpha, Mat<T> A, T beta, Mat<T> C, Mat<T> D, int64_t batch) {
+    return launch_syrk_ref<T>(s, alpha, A, beta, C, D, batch);
+}
+
+template <typename T>
+int dispatch_trmm(hipStream_t s, int side_right, T alpha, Mat<T> A, Mat<T> B, Mat<T> C, int64_t batch) {
+    return launch_trmm_ref<T>(s, side_right, alpha, A, B, C, batch);
+}
+
+template <typename T>
+int dispatch_trsm(hipStream_t s, int side_right, int upper, int unit, T alpha, Mat<T> A, Mat<T> B, Mat<T> X,
+                  int64_t batch) {
+    return launch_trsm_ref<T>(s, side_right, upper, unit, alpha, A, B, X, batch);
+}
+
+template <typename T>
+int dispatch_potrf(hipStream_t s, Mat<T> A, Mat<T> L, int64_t batch, int32_t* info) {
+    return launch_potrf_ref<T>(s, A, L, batch, info);
+}
+
+template <typename T>
+int dispatch_getrf(hipStream_t s, Mat<T> A, int64_t* ipiv, int64_t ipiv_stride, int64_t batch, int32_t* info) {
+    return launch_getrf_ref<T>(s, A, ipiv, ipiv_stride, batch, info);
+}
+
+template <typename T>
+int dispatch_posv(hipStream_t s, Mat<T> A, Mat<T> L, Mat<T> B, Mat<T> X, int64_t batch, int32_t* info) {
+    int r = launch_potrf_ref<T>(s, A, L, batch, info);
+    if (r) return r;
+    r = launch_trsm_ref<T>(s, 0, 0, 0, T(1), L, B, X, batch);
+    if (r) return r;
+    return launch_trsm_ref<T>(s, 0, 1, 0, T(1), transposed(L), X, X, batch);
+}
+
+#define INSTANTIATE(T)                                                                              \
+    template int dispatch_gemm<T>(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t);     \
+    template int dispatch_syrk<T>(hipStream_t, T, Mat<T>, T, Mat<T>, Mat<T>, int64_t);             \
+    template int dispatch_trmm<T>(hipStream_t, int, T, Mat<T>, Mat<T>, Mat<T>, int64_t);           \
+    template int dispatch_trsm<T>(hipStream_t, int, int, int, T, Mat<T>, Mat<T>, Mat<T>, int64_t); \
+    template int dispatch_potrf<T>(hipStream_t, Mat<T>, Mat<T>, int64_t, int32_t*);                \
+    template int dispatch_getrf<T>(hipStream_t, Mat<T>, int64_t*, int64_t, int64_t, int32_t*);     \
+    template int dispatch_posv<T>(hipStream_t, Mat<T>, Mat<T>, Mat<T>, Mat<T>, int64_t, int32_t*);
+INSTANTIATE(double)
+INSTANTIATE(float)
+
+}  // namespace blast_hip

@@ -1,0 +1,137 @@
+/*
+ * blast_hip.h - C ABI of the MI355X (gfx950) batched dense-kernel layer.
+ *
+ * Every entry point is the batched, device-side replacement of one BLAST
+ * function template (paths relative to the mkatliar/blast checkout).  The
+ * reference has no FFI: its boundary is C++20 templates instantiated in the
+ * caller's translation unit.  This header is what those templates' batched
+ * overloads (include/blast/hip/Blast.hpp) call; any other language binds it
+ * directly.
+ *
+ * Conventions
+ *  - All matrix data pointers are DEVICE pointers (hipMalloc'd or torch CUDA
+ *    tensors).  Descriptors, scalars and sizes are host values.
+ *  - A batch operand is `batch` matrices, `batch_stride` elements apart, each
+ *    stored column-major in BLAST's dense format (element (i,j) at
+ *    i + spacing*j; dense/DynamicMatrix.hpp:149) or panel format (element
+ *    (i,j) at (i/PS)*spacing + i%PS + j*PS, PS = 4 for double and 8 for float;
+ *    panel/DynamicPanelMatrix.hpp:184-188).  `rows` x `cols` is the LOGICAL
+ *    size; trans = 1 views the storage transposed (BLAST trans(); a row-major
+ *    matrix is the trans = 1 view of its transpose's column-major storage).
+ *  - `stream` is a hipStream_t (NULL = default stream).  Calls are
+ *    stream-ordered, do not synchronise and do not allocate.
+ *  - Return value: 0 on success; -k if argument k (1-based, `stream` = 1) is
+ *    invalid; BLAST_HIP_ERR_RUNTIME if the HIP runtime reported an error
+ *    (blast_hip_last_error() describes it).  Per-instance numerical status
+ *    goes to the optional device array `info` (int32 per instance): 0 = ok,
+ *    k > 0 = column k (1-based) had a non-positive (potrf) or exactly zero
+ *    (getrf) pivot.  Instances are independent: one bad instance never
+ *    affects another.
+ *  - Aliasing allowed as in BLAST: C == D (gemm, syrk), X == B (trsm),
+ *    L == A (potrf).
+ */
+#ifndef BLAST_HIP_H
+#define BLAST_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BLAST_HIP_ERR_RUNTIME (-1000)
+
+typedef struct blast_matrix_t {
+    void* data;           /* device pointer to instance 0 */
+    int64_t rows;         /* logical rows */
+    int64_t cols;         /* logical columns */
+    int64_t spacing;      /* dense: leading dimension; panel: elements between panels */
+    int64_t batch_stride; /* elements between consecutive instances */
+    int32_t panel;        /* 1 = panel storage, 0 = dense column-major storage */
+    int32_t trans;        /* 1 = logical matrix is the transpose of the storage */
+} blast_matrix_t;
+
+/* Algorithm selection for all entry points of the calling process.
+ *  BLAST_HIP_ALGO_AUTO      fastest kernel for the shape (MFMA tiles, fused
+ *                           kernels); results agree with BLAST to backward
+ *                           error c*n*eps.
+ *  BLAST_HIP_ALGO_REFERENCE column-sweep kernels that perform BLAST's
+ *                           per-element operation sequence (same FMA order,
+ *                           IEEE division and square root): bit-identical
+ *                           to BLAST's CPU results. */
+enum { BLAST_HIP_ALGO_AUTO = 0, BLAST_HIP_ALGO_REFERENCE = 1 };
+int blast_hip_set_algorithm(int algo);
+int blast_hip_get_algorithm(void);
+
+const char* blast_hip_last_error(void);
+const char* blast_hip_version(void);
+
+/* D := alpha*A*B + beta*C.  Replaces blast::gemm(alpha, A, B, beta, C, D)
+ * (include/blast/math/algorithm/Gemm.hpp:80-97) and gemm(A, B, C, D)
+ * (:113-118, alpha = beta = 1); the NT form is B with trans = 1
+ * (bench/blast/math/panel/StaticGemm.cpp:37). */
+int blast_hip_dgemm_batched(void* stream, double alpha, const blast_matrix_t* A, const blast_matrix_t* B,
+                            double beta, const blast_matrix_t* C, const blast_matrix_t* D, int64_t batch);
+int blast_hip_sgemm_batched(void* stream, float alpha, const blast_matrix_t* A, const blast_matrix_t* B,
+                            float beta, const blast_matrix_t* C, const blast_matrix_t* D, int64_t batch);
+
+/* D := alpha*A*A^T + beta*C, lower triangle only (i >= j written).
+ * Replaces blast::syrkLower (include/blast/math/dense/Syrk.hpp:15-145). */
+int blast_hip_dsyrk_lower_batched(void* stream, double alpha, const blast_matrix_t* A, double beta,
+                                  const blast_matrix_t* C, const blast_matrix_t* D, int64_t batch);
+int blast_hip_ssyrk_lower_batched(void* stream, float alpha, const blast_matrix_t* A, float beta,
+                                  const blast_matrix_t* C, const blast_matrix_t* D, int64_t batch);
+
+/* C := alpha*A*B, A upper triangular (side_right = 0), or C := alpha*B*A,
+ * A lower triangular (side_right = 1).  Replaces blast::trmm
+ * (include/blast/math/algorithm/Trmm.hpp:260-310; the other side/triangle
+ * combinations throw std::logic_error there and return -3 here). */
+int blast_hip_dtrmm_batched(void* stream, int side_right, int upper, double alpha, const blast_matrix_t* A,
+                            const blast_matrix_t* B, const blast_matrix_t* C, int64_t batch);
+int blast_hip_strmm_batched(void* stream, int side_right, int upper, float alpha, const blast_matrix_t* A,
+                            const blast_matrix_t* B, const blast_matrix_t* C, int64_t batch);
+
+/* Left (side_right = 0): A*X = alpha*B; right: X*A = alpha*B; A lower
+ * (upper = 0) or upper triangular, unit or non-unit diagonal.  X may alias B.
+ * Replaces blast::trsm<UpLo, UNIT>(A, B, X) (include/blast/math/dense/Trsm.hpp:17-66,
+ * alpha = 1) and the reference forms trsm(A, uplo, diag, X, alpha, B) /
+ * trsm(X, A, uplo, diag, alpha, B) (include/blast/math/reference/Trsm.hpp:164,206). */
+int blast_hip_dtrsm_batched(void* stream, int side_right, int upper, int unit, double alpha,
+                            const blast_matrix_t* A, const blast_matrix_t* B, const blast_matrix_t* X,
+                            int64_t batch);
+int blast_hip_strsm_batched(void* stream, int side_right, int upper, int unit, float alpha,
+                            const blast_matrix_t* A, const blast_matrix_t* B, const blast_matrix_t* X,
+                            int64_t batch);
+
+/* L := chol(A), lower, M >= N (tall allowed).  Replaces blast::potrf
+ * (include/blast/math/dense/Potrf.hpp:66-106 and panel/Potrf.hpp:59-106).
+ * Dense L: only i >= j written.  Panel L: the strict upper part of each 4x4
+ * diagonal block is written as 0.  Non-positive pivots produce NaN as in
+ * BLAST and are reported in info. */
+int blast_hip_dpotrf_batched(void* stream, const blast_matrix_t* A, const blast_matrix_t* L, int64_t batch,
+                             int32_t* info);
+int blast_hip_spotrf_batched(void* stream, const blast_matrix_t* A, const blast_matrix_t* L, int64_t batch,
+                             int32_t* info);
+
+/* In-place P*A = L*U with partial pivoting (column-major storage) or without
+ * pivoting (trans = 1, i.e. row-major storage).  ipiv: device int64 array,
+ * min(M,N) entries per instance, ipiv_stride apart, 0-based, LAPACK
+ * sequential-interchange meaning.  Replaces blast::getrf
+ * (include/blast/math/dense/Getrf.hpp:52-119 and :141-162). */
+int blast_hip_dgetrf_batched(void* stream, const blast_matrix_t* A, int64_t* ipiv, int64_t ipiv_stride,
+                             int64_t batch, int32_t* info);
+int blast_hip_sgetrf_batched(void* stream, const blast_matrix_t* A, int64_t* ipiv, int64_t ipiv_stride,
+                             int64_t batch, int32_t* info);
+
+/* Cholesky solve, the sequence potrf(A, L); trsm<Lower>(L, B, Y);
+ * trsm<Upper>(trans(L), Y, X) fused into one launch (LAPACK dposv contract:
+ * L and X are written).  X may alias B, L may alias A.  A is read from its
+ * lower triangle only. */
+int blast_hip_dposv_batched(void* stream, const blast_matrix_t* A, const blast_matrix_t* L,
+                            const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch, int32_t* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BLAST_HIP_H */
