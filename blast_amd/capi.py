@@ -45,6 +45,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+        # torch bundles its own HIP runtime and loads it as "libamdhip64.so";
+        # loading torch first lets libblast_hip.so's libamdhip64.so.7 /
+        # libhsa-runtime64.so.1 dependencies bind to that same runtime by
+        # SONAME, so the process holds one HIP runtime, not two.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         P = ctypes.POINTER(BlastMatrix)
         vp, i64, i32p, c_int = ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int

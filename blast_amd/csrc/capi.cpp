@@ -75,6 +75,7 @@ int gemm_entry(void* stream, T alpha, const blast_matrix_t* A, const blast_matri
     if (B->rows != A->cols) { set_error("gemm: Matrix sizes do not match (B)"); return -4; }
     if (C->rows != A->rows || C->cols != B->cols) { set_error("gemm: Matrix sizes do not match (C)"); return -6; }
     if (D->rows != A->rows || D->cols != B->cols) { set_error("gemm: Matrix sizes do not match (D)"); return -7; }
+    (void)hipGetLastError();  // drop a stale error left by an earlier, unrelated HIP call
     return dispatch_gemm<T>(hipStream_t(stream), alpha, Mat<T>::from(*A), Mat<T>::from(*B), beta,
                             Mat<T>::from(*C), Mat<T>::from(*D), batch);
 }
@@ -88,6 +89,7 @@ int syrk_entry(void* stream, T alpha, const blast_matrix_t* A, T beta, const bla
     if (!valid_batch(batch)) return -7;
     if (C->rows != A->rows || C->cols != A->rows) { set_error("syrk: Matrix sizes do not match (C)"); return -5; }
     if (D->rows != A->rows || D->cols != A->rows) { set_error("syrk: Matrix sizes do not match (D)"); return -6; }
+    (void)hipGetLastError();  // drop a stale error left by an earlier, unrelated HIP call
     return dispatch_syrk<T>(hipStream_t(stream), alpha, Mat<T>::from(*A), beta, Mat<T>::from(*C),
                             Mat<T>::from(*D), batch);
 }
@@ -107,6 +109,7 @@ int trmm_entry(void* stream, int side_right, int upper, T alpha, const blast_mat
     int64_t const na = side_right ? B->cols : B->rows;
     if (A->rows != na || A->cols != na) { set_error("trmm: Matrix sizes do not match (A)"); return -5; }
     if (C->rows != B->rows || C->cols != B->cols) { set_error("trmm: Matrix sizes do not match (C)"); return -7; }
+    (void)hipGetLastError();  // drop a stale error left by an earlier, unrelated HIP call
     return dispatch_trmm<T>(hipStream_t(stream), side_right, alpha, Mat<T>::from(*A), Mat<T>::from(*B),
                             Mat<T>::from(*C), batch);
 }
@@ -122,6 +125,7 @@ int trsm_entry(void* stream, int side_right, int upper, int unit, T alpha, const
     // dense/Trsm.hpp:244-251: "Invalid argument size"
     if (A->rows != na || A->cols != na) { set_error("trsm: Invalid argument size (A)"); return -6; }
     if (X->rows != B->rows || X->cols != B->cols) { set_error("trsm: Invalid argument size (X)"); return -8; }
+    (void)hipGetLastError();  // drop a stale error left by an earlier, unrelated HIP call
     return dispatch_trsm<T>(hipStream_t(stream), side_right, upper, unit, alpha, Mat<T>::from(*A),
                             Mat<T>::from(*B), Mat<T>::from(*X), batch);
 }
@@ -134,6 +138,7 @@ int potrf_entry(void* stream, const blast_matrix_t* A, const blast_matrix_t* L, 
     // dense/Potrf.hpp:78-85: "Invalid matrix size"
     if (A->cols > A->rows) { set_error("potrf: Invalid matrix size (columns > rows)"); return -2; }
     if (L->rows != A->rows || L->cols != A->cols) { set_error("potrf: Invalid matrix size (L)"); return -3; }
+    (void)hipGetLastError();  // drop a stale error left by an earlier, unrelated HIP call
     return dispatch_potrf<T>(hipStream_t(stream), Mat<T>::from(*A), Mat<T>::from(*L), batch, info);
 }
 
@@ -145,6 +150,7 @@ int getrf_entry(void* stream, const blast_matrix_t* A, int64_t* ipiv, int64_t ip
     if (k > 0 && !ipiv) { set_error("getrf: null ipiv"); return -3; }
     if (ipiv_stride < k) { set_error("getrf: ipiv_stride < min(M, N)"); return -4; }
     if (!valid_batch(batch)) return -5;
+    (void)hipGetLastError();  // drop a stale error left by an earlier, unrelated HIP call
     return dispatch_getrf<T>(hipStream_t(stream), Mat<T>::from(*A), ipiv, ipiv_stride, batch, info);
 }
 
@@ -160,6 +166,7 @@ int posv_entry(void* stream, const blast_matrix_t* A, const blast_matrix_t* L, c
     if (L->rows != A->rows || L->cols != A->cols) { set_error("posv: Invalid matrix size (L)"); return -3; }
     if (B->rows != A->rows) { set_error("posv: Invalid argument size (B)"); return -4; }
     if (X->rows != B->rows || X->cols != B->cols) { set_error("posv: Invalid argument size (X)"); return -5; }
+    (void)hipGetLastError();  // drop a stale error left by an earlier, unrelated HIP call
     return dispatch_posv<T>(hipStream_t(stream), Mat<T>::from(*A), Mat<T>::from(*L), Mat<T>::from(*B),
                             Mat<T>::from(*X), batch, info);
 }

@@ -3,6 +3,8 @@
 // and layout qualify.
 #include "dispatch.hpp"
 
+#include <type_traits>
+
 namespace blast_hip {
 
 template <typename T>
@@ -28,6 +30,13 @@ int dispatch_trsm(hipStream_t s, int side_right, int upper, int unit, T alpha, M
 
 template <typename T>
 int dispatch_potrf(hipStream_t s, Mat<T> A, Mat<T> L, int64_t batch, int32_t* info) {
+    if constexpr (std::is_same_v<T, double>) {
+        if (algorithm() == BLAST_HIP_ALGO_AUTO) {
+            Mat<double> none{};
+            int const r = launch_posv_tiles(s, A, L, none, none, batch, info);
+            if (r != 1) return r;
+        }
+    }
     return launch_potrf_ref<T>(s, A, L, batch, info);
 }
 
@@ -38,6 +47,12 @@ int dispatch_getrf(hipStream_t s, Mat<T> A, int64_t* ipiv, int64_t ipiv_stride, 
 
 template <typename T>
 int dispatch_posv(hipStream_t s, Mat<T> A, Mat<T> L, Mat<T> B, Mat<T> X, int64_t batch, int32_t* info) {
+    if constexpr (std::is_same_v<T, double>) {
+        if (algorithm() == BLAST_HIP_ALGO_AUTO && X.cols > 0) {
+            int const r = launch_posv_tiles(s, A, L, B, X, batch, info);
+            if (r != 1) return r;
+        }
+    }
     int r = launch_potrf_ref<T>(s, A, L, batch, info);
     if (r) return r;
     r = launch_trsm_ref<T>(s, 0, 0, 0, T(1), L, B, X, batch);

@@ -16,6 +16,9 @@ template <typename T> int launch_trsm_ref(hipStream_t, int, int, int, T, Mat<T>,
 template <typename T> int launch_potrf_ref(hipStream_t, Mat<T>, Mat<T>, int64_t, int32_t*);
 template <typename T> int launch_getrf_ref(hipStream_t, Mat<T>, int64_t*, int64_t, int64_t, int32_t*);
 
+// posv_tiles.hip: returns 1 when the shape is not covered
+int launch_posv_tiles(hipStream_t, Mat<double>, Mat<double>, Mat<double>, Mat<double>, int64_t, int32_t*);
+
 // dispatch.cpp
 template <typename T> int dispatch_gemm(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t);
 template <typename T> int dispatch_syrk(hipStream_t, T, Mat<T>, T, Mat<T>, Mat<T>, int64_t);

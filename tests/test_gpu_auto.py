@@ -1,0 +1,81 @@
+"""Fast (AUTO) kernels vs the oracle: agreement to backward error c*n*eps, plus
+the storage contract (what is and is not written).  GPU, through the C ABI."""
+
+import numpy as np
+import pytest
+
+from blast_amd import capi
+from blast_amd.layout import Layout
+from oracle import oracle
+from tests.numerics import EPS64, potrf_backward_error, solve_backward_error, spd
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def auto_algorithm():
+    capi.set_algorithm(capi.ALGO_AUTO)
+    yield
+
+
+def dev(layout, host):
+    return capi.Operand.from_host(layout, host)
+
+
+@pytest.mark.parametrize("n", [1, 3, 16, 17, 40, 64, 100, 127, 128])
+@pytest.mark.parametrize("nrhs", [1, 4])
+@pytest.mark.parametrize("panel", [True, False])
+def test_posv_fused_vs_oracle(n, nrhs, panel):
+    import torch
+
+    rng = np.random.default_rng(n * 10 + nrhs)
+    batch = 7
+    a = spd(rng, batch, n)
+    b = rng.uniform(-1, 1, (batch, n, nrhs))
+    la, lb = Layout.make(n, n, panel=panel), Layout.make(n, nrhs, panel=not panel)
+    ah, bh = la.pack(a), lb.pack(b)
+    lh = np.full_like(ah, 3.0)
+    xh = np.zeros_like(bh)
+    oracle.posv(la, ah, la, lh, lb, bh, lb, xh, batch)
+    L, X = capi.Operand.empty(la, batch, fill=3.0), capi.Operand.empty(lb, batch)
+    info = torch.full((batch,), -1, dtype=torch.int32, device="cuda")
+    capi.posv(dev(la, ah), L, dev(lb, bh), X, batch, info)
+    assert np.all(info.cpu().numpy() == 0)
+    x, x_ref = X.logical(batch), lb.unpack(xh, batch)
+    lg, l_ref = L.logical(batch), la.unpack(lh, batch)
+    assert np.all(solve_backward_error(a, x, b) < 4 * n * EPS64)
+    assert np.all(potrf_backward_error(a, lg) < 4 * n * EPS64)
+    np.testing.assert_allclose(x, x_ref, rtol=0, atol=1e-13 * np.abs(x_ref).max())
+    low = np.tril(np.ones((n, n), bool))
+    np.testing.assert_allclose(lg[:, low], l_ref[:, low], rtol=0, atol=1e-13 * np.abs(l_ref).max())
+    # storage contract: identical to the reference-order path outside the lower triangle
+    np.testing.assert_array_equal(lg[:, ~low], l_ref[:, ~low])
+
+
+@pytest.mark.parametrize("n", [8, 128])
+def test_potrf_fused_inplace_and_info(n):
+    import torch
+
+    rng = np.random.default_rng(3)
+    batch = 6
+    a = spd(rng, batch, n)
+    a[4, n // 2, n // 2] = -1.0e4  # not SPD
+    la = Layout.make(n, n)
+    A = dev(la, la.pack(a))
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    capi.potrf(A, A, batch, info)  # L aliases A
+    inf = info.cpu().numpy()
+    assert inf[4] > 0 and np.all(np.delete(inf, 4) == 0)
+    good = [0, 1, 2, 3, 5]
+    assert np.all(potrf_backward_error(a[good], A.logical(batch)[good]) < 4 * n * EPS64)
+
+
+def test_posv_alias_x_b():
+    rng = np.random.default_rng(4)
+    n, batch = 96, 5
+    a = spd(rng, batch, n)
+    b = rng.uniform(-1, 1, (batch, n, 3))
+    la, lb = Layout.make(n, n), Layout.make(n, 3)
+    A, B = dev(la, la.pack(a)), dev(lb, lb.pack(b))
+    capi.posv(A, A, B, B, batch)  # L aliases A, X aliases B
+    assert np.all(solve_backward_error(a, B.logical(batch), b) < 4 * n * EPS64)
