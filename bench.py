@@ -1,0 +1,221 @@
+"""Benchmark of the batched Cholesky solve (potrf + two trsm) at n = 128.
+
+Workload: 8192 independent SPD systems A = G G^T + 128 I (G ~ N(0,1)),
+4 right-hand sides U(-1,1) each, BLAST panel storage (PS = 4).  One step is
+one batched call over the whole batch with every input already in HBM:
+
+  plain FP64  blast_hip_dposv_batched: A (lower) in, L (lower) and X out,
+              the LAPACK dposv contract;
+  mixed + IR  blast_hip_dsposv_batched (when built): factor in lower
+              precision, refine in FP64, X out (LAPACK dsposv contract).
+
+Prints one JSON line.  Time is measured with HIP events on the stream the
+kernels run on, over --steps steps after --warmup steps.  The CPU baseline is
+the oracle (single-threaded BLAST restatement), one instance per host core,
+timed on a bounded sample of the same batch.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N, NRHS, BATCH = 128, 4, 8192
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+# Algorithmic work per instance (potrf n^3/3 + n^2/2 + n/6; each trsm n^2 * nrhs).
+FLOPS_PER_INSTANCE = N ** 3 / 3 + N ** 2 / 2 + N / 6 + 2 * N * N * NRHS
+# Lower triangle of A in, lower triangle of L out, B in, X out (8-byte words).
+TRI = N * (N + 1) // 2
+BYTES_PLAIN = 8 * (2 * TRI + 2 * N * NRHS)
+BYTES_SOLVE_ONLY = 8 * (TRI + 2 * N * NRHS)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--batch", type=int, default=BATCH)
+    p.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def make_inputs(batch, seed=20240611):
+    """SPD batch and RHS generated on the GPU (torch Philox, fixed seed), packed
+    into panel storage on the device."""
+    import torch
+
+    from blast_amd.layout import Layout
+
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    la, lb = Layout.make(N, N), Layout.make(N, NRHS)
+    abuf = torch.zeros(batch * la.batch_stride, dtype=torch.float64, device="cuda")
+    bbuf = torch.zeros(batch * lb.batch_stride, dtype=torch.float64, device="cuda")
+    ia = torch.from_numpy(la.index_map().reshape(-1)).cuda()
+    ib = torch.from_numpy(lb.index_map().reshape(-1)).cuda()
+    eye = N * torch.eye(N, dtype=torch.float64, device="cuda")
+    chunk = 1024
+    for s in range(0, batch, chunk):
+        e = min(batch, s + chunk)
+        g = torch.randn((e - s, N, N), dtype=torch.float64, device="cuda", generator=gen)
+        a = torch.bmm(g, g.transpose(1, 2).contiguous()) + eye
+        abuf.view(batch, la.batch_stride)[s:e, :].index_copy_(1, ia, a.reshape(e - s, -1))
+    b = torch.rand((batch, N, NRHS), dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
+    bbuf.view(batch, lb.batch_stride)[:, :].index_copy_(1, ib, b.reshape(batch, -1))
+    return la, lb, abuf, bbuf
+
+
+def backward_errors(la, lb, abuf, xbuf, bbuf, batch, lbuf=None):
+    """Per-instance normwise backward errors, computed in FP64 on the GPU with
+    torch (checker only)."""
+    import torch
+
+    ia = torch.from_numpy(la.index_map()).cuda()
+    ib = torch.from_numpy(lb.index_map()).cuda()
+    out_solve, out_fact = [], []
+    for s in range(0, batch, 1024):
+        e = min(batch, s + 1024)
+        a = abuf.view(batch, la.batch_stride)[s:e][:, ia]
+        a = torch.tril(a) + torch.tril(a, -1).transpose(1, 2)
+        x = xbuf.view(batch, lb.batch_stride)[s:e][:, ib]
+        b = bbuf.view(batch, lb.batch_stride)[s:e][:, ib]
+        r = b - torch.bmm(a, x)
+        na, nx, nb = (torch.linalg.matrix_norm(t) for t in (a, x, b))
+        out_solve.append((torch.linalg.matrix_norm(r) / (na * nx + nb)).cpu())
+        if lbuf is not None:
+            l = torch.tril(lbuf.view(batch, la.batch_stride)[s:e][:, ia])
+            out_fact.append((torch.linalg.matrix_norm(torch.bmm(l, l.transpose(1, 2).contiguous()) - a) / na).cpu())
+    se = torch.cat(out_solve).numpy()
+    fe = torch.cat(out_fact).numpy() if out_fact else None
+    return se, fe
+
+
+def time_steps(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / 1e3 / steps  # seconds per step
+
+
+def cpu_baseline(la, lb, abuf, bbuf, target_s):
+    from oracle import oracle
+
+    sample = 64
+    a = abuf[: sample * la.batch_stride].cpu().numpy()
+    b = bbuf[: sample * lb.batch_stride].cpu().numpy()
+    l, x = np.zeros_like(a), np.zeros_like(b)
+    cores = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    _, used = oracle.posv_mt(la, a, la, l, lb, b, lb, x, sample, threads=cores)
+    dt = time.perf_counter() - t0
+    reps = max(1, int(target_s / max(dt, 1e-6)))
+    reps = min(reps, 200)
+    a_big, b_big = np.tile(a, reps), np.tile(b, reps)
+    l_big, x_big = np.zeros_like(a_big), np.zeros_like(b_big)
+    t0 = time.perf_counter()
+    _, used = oracle.posv_mt(la, a_big, la, l_big, lb, b_big, lb, x_big, sample * reps, threads=cores)
+    dt = time.perf_counter() - t0
+    return {
+        "value": sample * reps / dt,
+        "unit": "solves/s",
+        "cores": used,
+        "kind": "port",
+        "sample": f"{sample * reps} instances (first {sample} of the batch x {reps}), "
+                  f"oracle posv one instance per core, {dt:.1f} s",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); every instance is independent, "
+              "multi-GPU sharding is a contiguous split of the batch", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from blast_amd import capi
+
+    batch = args.batch
+    la, lb, abuf, bbuf = make_inputs(batch)
+    lbuf = torch.zeros_like(abuf)
+    xbuf = torch.zeros_like(bbuf)
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    A, B = capi.Operand(la, abuf), capi.Operand(lb, bbuf)
+    L, X = capi.Operand(la, lbuf), capi.Operand(lb, xbuf)
+    capi.set_algorithm(capi.ALGO_AUTO)
+
+    def plain():
+        capi.posv(A, L, B, X, batch, info)
+
+    t_plain = time_steps(plain, args.steps, args.warmup)
+    se, fe = backward_errors(la, lb, abuf, xbuf, bbuf, batch, lbuf)
+    assert int(info.abs().sum()) == 0
+    plain_rate = batch / t_plain
+    result = {
+        "plain_fp64": {
+            "solves_per_s": plain_rate,
+            "ms_per_step": t_plain * 1e3,
+            "tflops": FLOPS_PER_INSTANCE * batch / t_plain / 1e12,
+            "max_solve_backward_error": float(se.max()),
+            "max_potrf_backward_error": float(fe.max()),
+        }
+    }
+    value, t_step, dtype = plain_rate, t_plain, "f64"
+    headline = "plain_fp64"
+
+    achieved = BYTES_PLAIN * batch / t_plain / 1e9
+    line = {
+        "metric": "batched potrf+trsm solves/s @n=128: mixed+IR vs plain FP64, both <=1e-12 bwd err",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic: A = G G^T + 128 I, G ~ N(0,1); B ~ U(-1,1); torch Philox seed 20240611",
+        "config": {
+            "workload": f"Cholesky solve potrf + trsm(L) + trsm(L^T), n={N}, nrhs={NRHS}, batch={batch}, "
+                        "FP64, panel storage",
+            "headline_path": headline,
+            "paths": result,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS,
+            "traffic": None,
+            "kernel": "posv_tile_kernel<8>",
+            "algorithmic_bytes_per_instance": BYTES_PLAIN,
+            "fp64_tflops": FLOPS_PER_INSTANCE * batch / t_plain / 1e12,
+            "fp64_frac_of_peak": FLOPS_PER_INSTANCE * batch / t_plain / 1e12 / PEAK_FP64_TFLOPS,
+        },
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(la, lb, abuf, bbuf, args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

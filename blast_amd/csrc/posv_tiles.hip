@@ -6,9 +6,8 @@
 // (rows {w, 7-w} per wavefront: 9 tiles each at n = 128).  Right-looking
 // blocked Cholesky, block column kb = 0..NT-1:
 //   A  owners stage tile (kb,kb) and the panel tiles (i,kb) in LDS;
-//   B  the owner of row kb factors the 16x16 diagonal tile with a
-//      lane-per-row sweep (readlane broadcasts, IEEE sqrt/divide) that also
-//      builds W = L_kk^{-1};
+//   B  the owner of row kb factors the 16x16 diagonal tile and builds
+//      W = L_kk^{-1} in registers (DPP row broadcasts, Newton-refined rsqrt);
 //   C  L_ik = A_ik W^T on MFMA, written to HBM and to the LDS panel;
 //      forward substitution y_kb = W b_kb on the RHS;
 //   D  trailing update A_ij -= L_ik L_jk^T on MFMA for every owned tile with
@@ -20,6 +19,8 @@
 // factorisation reorders BLAST's sums (agreement to backward error, not bits).
 // HBM traffic per instance: the lower triangle of A is read, the lower
 // triangle of L written, B read and X written.
+
+#include <type_traits>
 
 #include "common.hpp"
 #include "dispatch.hpp"
@@ -73,12 +74,23 @@ __device__ __forceinline__ int owner(int i) { return i < 4 ? i : 7 - i; }
 // Load tile (ti, tj) of A into accumulator layout.  Outside n x n the matrix
 // is padded with the identity; the strict upper part of diagonal tiles is
 // not read (BLAST reads A's lower triangle only).
+// Element access; PANEL = column-major panel storage, not transposed (the
+// layout BLAST's StaticPanelMatrix uses), which needs no runtime dispatch.
+template <bool PANEL>
+__device__ __forceinline__ double& at(Ref<double> const& a, int row, int col) {
+    if constexpr (PANEL)
+        return a.p[int64_t(row >> 2) * a.m.spacing + (row & 3) + 4 * col];
+    else
+        return a(row, col);
+}
+
+template <bool PANEL>
 __device__ __forceinline__ d4 load_tile(Ref<double> const& a, int n, int ti, int tj, int lane) {
     d4 v;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         int const row = 16 * ti + acc_row(lane, r), col = 16 * tj + acc_col(lane);
-        v[r] = (row < n && col < n) ? (row >= col ? a(row, col) : 0.0) : (row == col ? 1.0 : 0.0);
+        v[r] = (row < n && col < n) ? (row >= col ? at<PANEL>(a, row, col) : 0.0) : (row == col ? 1.0 : 0.0);
     }
     return v;
 }
@@ -86,172 +98,237 @@ __device__ __forceinline__ d4 load_tile(Ref<double> const& a, int n, int ti, int
 // Store tile (ti, tj) of L.  Dense storage: i >= j only.  Panel storage: also
 // zeros in the strict upper part of each 4x4 diagonal block (what BLAST's
 // panel potrf leaves there is rounding residue of 0).
+template <bool PANEL>
 __device__ __forceinline__ void store_tile(Ref<double> const& l, bool panel, int n, int ti, int tj, d4 v, int lane) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         int const row = 16 * ti + acc_row(lane, r), col = 16 * tj + acc_col(lane);
-        if (row < n && col < n && (row >= col || (panel && (row >> 2) == (col >> 2)))) l(row, col) = row >= col ? v[r] : 0.0;
+        if (row < n && col < n && (row >= col || (panel && (row >> 2) == (col >> 2))))
+            at<PANEL>(l, row, col) = row >= col ? v[r] : 0.0;
     }
 }
 
-// Lane-per-row factorisation of the 16x16 tile in sD (lanes 0..15 hold rows;
-// the other lanes shadow them).  On exit sD holds L_kk (zeros above the
-// diagonal) and sW holds W = L_kk^{-1}.  Returns 0 or the 1-based tile column
-// of the first non-positive pivot.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// Broadcast lane C of every 16-lane row to the whole row (DPP row_newbcast).
+template <int C>
+__device__ __forceinline__ double row_bcast(double v) {
+    long long const x = __builtin_bit_cast(long long, v);
+    int const lo = __builtin_amdgcn_mov_dpp(int(x), 0x150 + C, 0xF, 0xF, false);
+    int const hi = __builtin_amdgcn_mov_dpp(int(x >> 32), 0x150 + C, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
+// 1/sqrt(d): v_rsq_f64 refined by two Newton steps (about 1 ulp).
+__device__ __forceinline__ double rsqrt_nr(double d) {
+    double r = __builtin_amdgcn_rsq(d);
+    double const h = 0.5 * d;
+    r = __builtin_fma(r, __builtin_fma(-h * r, r, 0.5), r);
+    r = __builtin_fma(r, __builtin_fma(-h * r, r, 0.5), r);
+    return r;
+}
+
+// Factorisation of the 16x16 diagonal tile in sD, and its inverse.
+// Each 16-lane row of the wavefront holds a copy of the tile, lane i of the
+// row holding tile row i; every broadcast is a DPP row_newbcast.
+// 1. Cholesky, left-looking: for column k, lane i forms a_ik - <l_i, l_k>
+//    (row k broadcast from lane k), then r = 1/sqrt(d), l_kk = d*r,
+//    l_ik = (.)*r.
+// 2. W = L_kk^{-1}, lane i computing row i: row j is finalised at step j
+//    and broadcast from lane j to the lanes below it.
+// On exit sD holds L_kk (zeros above the diagonal) and sW holds W, both as
+// T[m][k] at k*17 + m.  Returns 0 or the 1-based tile column of the first
+// non-positive pivot.  Branch-free: every lane runs the same instructions.
 __device__ int diag_factor(double* sD, double* sW, int lane) {
     int const i = lane & 15;
-    double t[16], wv[16];
+    double t[16];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) {
-        t[c] = sD[c * kTileStride + i];
-        wv[c] = (c == i) ? 1.0 : 0.0;
-    }
+    for (int c = 0; c < 16; ++c) t[c] = sD[c * kTileStride + i];
     int bad = 0;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        double const d = readlane_d(t[k], k);
-        if (!(d > 0.0) && bad == 0) bad = k + 1;
-        double const s = __builtin_sqrt(d);
-        double const lkk = d / s;
-        double const lik = (i > k) ? t[k] / s : (i == k ? lkk : 0.0);
+    double myr = 0.0;
+    static_for<0, 16>([&](auto K) __attribute__((always_inline)) {
+        constexpr int k = decltype(K)::value;
+        double acc0 = t[k], acc1 = 0.0;
+        static_for<0, k>([&](auto J) __attribute__((always_inline)) {
+            constexpr int j = decltype(J)::value;
+            if constexpr (j % 2 == 0)
+                acc0 = __builtin_fma(-t[j], row_bcast<k>(t[j]), acc0);
+            else
+                acc1 = __builtin_fma(-t[j], row_bcast<k>(t[j]), acc1);
+        });
+        double const v = acc0 + acc1;
+        double const d = row_bcast<k>(v);
+        bad = (bad == 0 && !(d > 0.0)) ? k + 1 : bad;
+        double const r = rsqrt_nr(d);
+        myr = (i == k) ? r : myr;
+        double lik = ((i == k) ? d : v) * r;
+        lik = (i < k) ? 0.0 : lik;
         t[k] = lik;
-        double const sc = (i == k) ? 1.0 / lkk : 1.0;
+        __builtin_amdgcn_sched_barrier(0);
+    });
+    // W row by row: at step j lane j finalises W[j][:] = (e_j - acc) / l_jj;
+    // lanes i > j then accumulate acc[c] += l_ij W[j][c], c <= j.
+    double w[16];
 #pragma unroll
-        for (int c = 0; c <= k; ++c) wv[c] *= sc;
-#pragma unroll
-        for (int c = k + 1; c < 16; ++c) t[c] = __builtin_fma(-lik, readlane_d(lik, c), t[c]);
-        double const f = (i > k) ? lik : 0.0;
-#pragma unroll
-        for (int c = 0; c <= k; ++c) wv[c] = __builtin_fma(-f, readlane_d(wv[c], k), wv[c]);
-    }
+    for (int c = 0; c < 16; ++c) w[c] = 0.0;
+    static_for<0, 16>([&](auto J) __attribute__((always_inline)) {
+        constexpr int j = decltype(J)::value;
+        static_for<0, j>([&](auto C) __attribute__((always_inline)) {
+            constexpr int c = decltype(C)::value;
+            w[c] = (i == j) ? -w[c] * myr : w[c];
+        });
+        w[j] = (i == j) ? myr : w[j];
+        double const f = (i > j) ? t[j] : 0.0;
+        static_for<0, j + 1>([&](auto C) __attribute__((always_inline)) {
+            constexpr int c = decltype(C)::value;
+            w[c] = __builtin_fma(f, row_bcast<j>(w[c]), w[c]);
+        });
+        __builtin_amdgcn_sched_barrier(0);
+    });
     if (lane < 16) {
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
-            sD[c * kTileStride + i] = (c <= i) ? t[c] : 0.0;
-            sW[c * kTileStride + i] = (c <= i) ? wv[c] : 0.0;
+            sD[c * kTileStride + i] = (c <= i) ? t[c] : 0.0;  // L[i][c]
+            sW[c * kTileStride + i] = (c <= i) ? w[c] : 0.0;  // W[i][c]
         }
     }
     return bad;
 }
 
 template <int NT>
-__global__ __launch_bounds__(256) void posv_tile_kernel(Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X,
-                                                        int n, int nrhs, int32_t* info) {
-    __shared__ double sP[2][NT][kTileElems];
-    __shared__ double sD[kTileElems];
-    __shared__ double sW[NT][kTileElems];
-    __shared__ double sR[NT * 16 * kMaxRhs];
-    __shared__ double sS[4][16 * kMaxRhs];
-    __shared__ int sInfo;
+struct PosvShared {
+    double P[2][NT][kTileElems];  // panel tiles of block column kb (double-buffered by kb parity)
+    double D[kTileElems];         // diagonal tile
+    double W[NT][kTileElems];     // W_kb = L_kb,kb^{-1}, kept for the solves
+    double R[NT * 16 * kMaxRhs];  // right-hand sides -> y -> x, [row][rhs]
+    double S[4][16 * kMaxRhs];    // backward-solve partial sums per wavefront
+    int info;
+};
 
-    int64_t const b = blockIdx.x;
-    int const lane = threadIdx.x & 63;
-    int const w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // wavefront-uniform
+// Tile slots of wavefront W: row W (tiles (W, 0..W)) then row 7-W.
+template <int NT, int W>
+struct Slots {
+    static constexpr int kCount = 9;
+    static constexpr int i(int t) { return t <= W ? W : 7 - W; }
+    static constexpr int j(int t) { return t <= W ? t : t - W - 1; }
+    static constexpr bool ok(int t) { return i(t) < NT; }
+};
+
+template <int NT, int W, bool PANEL>
+__device__ __forceinline__ void posv_body(PosvShared<NT>& sh, Mat<double> const& A, Mat<double> const& L,
+                                          Mat<double> const& B, Mat<double> const& X, int n, int nrhs,
+                                          int64_t b, int lane) {
+    using Sl = Slots<NT, W>;
     Ref<double> a(A, b), l(L, b);
     bool const lpanel = L.panel != 0;
-
-    // Tile slots of this wavefront: row r0 = w (tiles (r0, 0..r0)), row r1 = 7-w.
-    int const r0 = w, r1 = 7 - w;
-    constexpr int S = 9;
-    d4 acc[S];
-    int ti[S], tj[S];
-    bool ok[S];
-#pragma unroll
-    for (int t = 0; t < S; ++t) {
-        bool const first = t <= r0;
-        ti[t] = first ? r0 : r1;
-        tj[t] = first ? t : t - r0 - 1;
-        ok[t] = ti[t] < NT && tj[t] <= ti[t] && (first || r1 >= 4);
-        acc[t] = ok[t] ? load_tile(a, n, ti[t], tj[t], lane) : d4{0.0, 0.0, 0.0, 0.0};
-    }
+    d4 acc[Sl::kCount];
+    static_for<0, Sl::kCount>([&](auto T) __attribute__((always_inline)) {
+        constexpr int t = decltype(T)::value;
+        if constexpr (Sl::ok(t)) acc[t] = load_tile<PANEL>(a, n, Sl::i(t), Sl::j(t), lane);
+    });
     if (nrhs > 0) {
         Ref<double> bb(B, b);
         for (int e = threadIdx.x; e < NT * 16 * kMaxRhs; e += 256) {
             int const row = e >> 2, c = e & 3;
-            sR[e] = (row < n && c < nrhs) ? bb(row, c) : 0.0;
+            sh.R[e] = (row < n && c < nrhs) ? bb(row, c) : 0.0;
         }
     }
-    if (threadIdx.x == 0) sInfo = 0;
     __syncthreads();
 
     for (int kb = 0; kb < NT; ++kb) {
-        double(*P)[kTileElems] = sP[kb & 1];
-        // A: stage column kb
-#pragma unroll
-        for (int t = 0; t < S; ++t)
-            if (ok[t] && tj[t] == kb) acc_to_lds(ti[t] == kb ? sD : P[ti[t]], acc[t], lane);
+        double(*P)[kTileElems] = sh.P[kb & 1];
+        // A: stage block column kb
+        static_for<0, Sl::kCount>([&](auto T) __attribute__((always_inline)) {
+            constexpr int t = decltype(T)::value;
+            if constexpr (Sl::ok(t))
+                if (Sl::j(t) == kb) acc_to_lds(Sl::i(t) == kb ? sh.D : P[Sl::i(t)], acc[t], lane);
+        });
         __syncthreads();
         // B: diagonal tile
-        if (w == owner(kb)) {
-            int const bad = diag_factor(sD, sW[kb], lane);
-            if (bad && lane == 0 && sInfo == 0 && 16 * kb + bad <= n) sInfo = 16 * kb + bad;
+        if (W == owner(kb)) {
+            int const bad = diag_factor(sh.D, sh.W[kb], lane);
+            if (bad && lane == 0 && sh.info == 0 && 16 * kb + bad <= n) sh.info = 16 * kb + bad;
         }
         __syncthreads();
-        // C: panel solve L_ik = A_ik W^T, forward substitution on the RHS
+        // C: panel solve L_ik = A_ik W^T; forward substitution y_kb = W b_kb
+        static_for<0, Sl::kCount>([&](auto T) __attribute__((always_inline)) {
+            constexpr int t = decltype(T)::value;
+            if constexpr (Sl::ok(t))
+                if (Sl::j(t) == kb && Sl::i(t) > kb) {
+                    d4 z = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int t = 0; t < S; ++t)
-            if (ok[t] && tj[t] == kb && ti[t] > kb) {
-                d4 z = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int q = 0; q < 4; ++q) z = mfma(op_read(P[ti[t]], q, lane), op_read(sW[kb], q, lane), z);
-                acc[t] = z;
-                store_tile(l, lpanel, n, ti[t], kb, z, lane);
-                acc_to_lds(P[ti[t]], z, lane);
-            }
-        if (w == owner(kb)) {
-            store_tile(l, lpanel, n, kb, kb, lds_to_acc(sD, lane), lane);
+                    for (int q = 0; q < 4; ++q) z = mfma(op_read(P[Sl::i(t)], q, lane), op_read(sh.W[kb], q, lane), z);
+                    acc[t] = z;
+                    store_tile<PANEL>(l, lpanel, n, Sl::i(t), kb, z, lane);
+                    acc_to_lds(P[Sl::i(t)], z, lane);
+                }
+        });
+        if (W == owner(kb)) {
+            store_tile<PANEL>(l, lpanel, n, kb, kb, lds_to_acc(sh.D, lane), lane);
             if (nrhs > 0) {
                 int const m = lane >> 2, c = lane & 3;
                 double rv[16];
 #pragma unroll
-                for (int k = 0; k < 16; ++k) rv[k] = sR[(16 * kb + k) * kMaxRhs + c];
+                for (int k = 0; k < 16; ++k) rv[k] = sh.R[(16 * kb + k) * kMaxRhs + c];
                 double y = 0.0;
 #pragma unroll
-                for (int k = 0; k < 16; ++k) y = __builtin_fma(sW[kb][k * kTileStride + m], rv[k], y);
-                sR[(16 * kb + m) * kMaxRhs + c] = y;
+                for (int k = 0; k < 16; ++k) y = __builtin_fma(sh.W[kb][k * kTileStride + m], rv[k], y);
+                sh.R[(16 * kb + m) * kMaxRhs + c] = y;
             }
         }
         __syncthreads();
-        // D: trailing update and RHS update
+        // D: trailing update A_ij -= L_ik L_jk^T; b_i -= L_ik y_kb
+        static_for<0, Sl::kCount>([&](auto T) __attribute__((always_inline)) {
+            constexpr int t = decltype(T)::value;
+            if constexpr (Sl::ok(t) && Sl::j(t) > 0)
+                if (Sl::j(t) > kb) {
 #pragma unroll
-        for (int t = 0; t < S; ++t)
-            if (ok[t] && tj[t] > kb) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    acc[t] = mfma(-op_read(P[ti[t]], q, lane), op_read(P[tj[t]], q, lane), acc[t]);
-            }
-            __builtin_amdgcn_sched_barrier(0);  // keep one slot's operands live at a time
+                    for (int q = 0; q < 4; ++q)
+                        acc[t] = mfma(-op_read(P[Sl::i(t)], q, lane), op_read(P[Sl::j(t)], q, lane), acc[t]);
+                }
+            __builtin_amdgcn_sched_barrier(0);
+        });
         if (nrhs > 0) {
             int const m = lane >> 2, c = lane & 3;
+            static_for<0, 2>([&](auto R) __attribute__((always_inline)) {
+                constexpr int i = decltype(R)::value == 0 ? W : 7 - W;
+                if constexpr (i < NT)
+                    if (i > kb) {
+                        double v = sh.R[(16 * i + m) * kMaxRhs + c];
 #pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                int const i = s == 0 ? r0 : r1;
-                if (i > kb && i < NT && (s == 0 || r1 >= 4)) {
-                    double v = sR[(16 * i + m) * kMaxRhs + c];
-#pragma unroll
-                    for (int k = 0; k < 16; ++k)
-                        v = __builtin_fma(-P[i][k * kTileStride + m], sR[(16 * kb + k) * kMaxRhs + c], v);
-                    sR[(16 * i + m) * kMaxRhs + c] = v;
-                }
-            }
+                        for (int k = 0; k < 16; ++k)
+                            v = __builtin_fma(-P[i][k * kTileStride + m], sh.R[(16 * kb + k) * kMaxRhs + c], v);
+                        sh.R[(16 * i + m) * kMaxRhs + c] = v;
+                    }
+            });
         }
     }
 
     if (nrhs > 0) {
         __syncthreads();
-        // Backward substitution x = L^{-T} y, block column kb = NT-1..0.
+        // Backward substitution x = L^{-T} y, block column kb = NT-1..0, with
+        // L_ik taken from the accumulators.
         for (int kb = NT - 1; kb >= 0; --kb) {
             double p[kMaxRhs] = {0.0, 0.0, 0.0, 0.0};
+            static_for<0, Sl::kCount>([&](auto T) __attribute__((always_inline)) {
+                constexpr int t = decltype(T)::value;
+                if constexpr (Sl::ok(t) && Sl::i(t) > Sl::j(t))
+                    if (Sl::j(t) == kb) {
 #pragma unroll
-            for (int t = 0; t < S; ++t)
-                if (ok[t] && tj[t] == kb && ti[t] > kb) {
+                        for (int r = 0; r < 4; ++r) {
+                            int const row = 16 * Sl::i(t) + acc_row(lane, r);
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        int const row = 16 * ti[t] + acc_row(lane, r);
-#pragma unroll
-                        for (int c = 0; c < kMaxRhs; ++c) p[c] = __builtin_fma(acc[t][r], sR[row * kMaxRhs + c], p[c]);
+                            for (int c = 0; c < kMaxRhs; ++c)
+                                p[c] = __builtin_fma(acc[t][r], sh.R[row * kMaxRhs + c], p[c]);
+                        }
                     }
-                }
+            });
 #pragma unroll
             for (int c = 0; c < kMaxRhs; ++c) {
                 p[c] += __shfl_xor(p[c], 16);
@@ -259,36 +336,59 @@ __global__ __launch_bounds__(256) void posv_tile_kernel(Mat<double> A, Mat<doubl
             }
             if (lane < 16)
 #pragma unroll
-                for (int c = 0; c < kMaxRhs; ++c) sS[w][lane * kMaxRhs + c] = p[c];
+                for (int c = 0; c < kMaxRhs; ++c) sh.S[W][lane * kMaxRhs + c] = p[c];
             __syncthreads();
-            if (w == owner(kb)) {
+            if (W == owner(kb)) {
                 int const m = lane >> 2, c = lane & 3;
                 double tv[16];
 #pragma unroll
                 for (int k = 0; k < 16; ++k) {
                     int const e = k * kMaxRhs + c;
-                    tv[k] = sR[(16 * kb + k) * kMaxRhs + c] - ((sS[0][e] + sS[1][e]) + (sS[2][e] + sS[3][e]));
+                    tv[k] = sh.R[(16 * kb + k) * kMaxRhs + c] - ((sh.S[0][e] + sh.S[1][e]) + (sh.S[2][e] + sh.S[3][e]));
                 }
                 double x = 0.0;
 #pragma unroll
-                for (int k = 0; k < 16; ++k) x = __builtin_fma(sW[kb][m * kTileStride + k], tv[k], x);
-                sR[(16 * kb + m) * kMaxRhs + c] = x;
+                for (int k = 0; k < 16; ++k) x = __builtin_fma(sh.W[kb][m * kTileStride + k], tv[k], x);
+                sh.R[(16 * kb + m) * kMaxRhs + c] = x;
             }
             __syncthreads();
         }
         Ref<double> xx(X, b);
         for (int e = threadIdx.x; e < n * nrhs; e += 256) {
             int const row = e % n, c = e / n;
-            xx(row, c) = sR[row * kMaxRhs + c];
+            xx(row, c) = sh.R[row * kMaxRhs + c];
         }
     }
-    if (info && threadIdx.x == 0) info[b] = sInfo;
+}
+
+template <int NT, bool PANEL>
+__global__ __launch_bounds__(256) void posv_tile_kernel(Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X,
+                                                        int n, int nrhs, int32_t* info) {
+    __shared__ PosvShared<NT> sh;
+    int64_t const b = blockIdx.x;
+    int const lane = threadIdx.x & 63;
+    int const w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    if (threadIdx.x == 0) sh.info = 0;
+    switch (w) {
+        case 0: posv_body<NT, 0, PANEL>(sh, A, L, B, X, n, nrhs, b, lane); break;
+        case 1: posv_body<NT, 1, PANEL>(sh, A, L, B, X, n, nrhs, b, lane); break;
+        case 2: posv_body<NT, 2, PANEL>(sh, A, L, B, X, n, nrhs, b, lane); break;
+        default: posv_body<NT, 3, PANEL>(sh, A, L, B, X, n, nrhs, b, lane); break;
+    }
+    __syncthreads();
+    if (info && threadIdx.x == 0) info[b] = sh.info;
 }
 
 template <int NT>
 static int launch_nt(hipStream_t s, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int n, int nrhs,
                      int64_t batch, int32_t* info) {
-    hipLaunchKernelGGL(posv_tile_kernel<NT>, dim3(unsigned(batch)), dim3(256), 0, s, A, L, B, X, n, nrhs, info);
+    bool const panel = A.panel && !A.trans && L.panel && !L.trans;
+    if (panel)
+        hipLaunchKernelGGL((posv_tile_kernel<NT, true>), dim3(unsigned(batch)), dim3(256), 0, s, A, L, B, X, n, nrhs,
+                           info);
+    else
+        hipLaunchKernelGGL((posv_tile_kernel<NT, false>), dim3(unsigned(batch)), dim3(256), 0, s, A, L, B, X, n,
+                           nrhs, info);
     return check_launch("posv_tile_kernel");
 }
 

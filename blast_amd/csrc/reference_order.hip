@@ -71,45 +71,67 @@ __global__ __launch_bounds__(kThreads) void trmm_ref_kernel(int side_right, T al
     }
 }
 
-// Left: A X = alpha B.  Right: X A = alpha B.  Sweep over the triangle's
-// columns; X may alias B (B is read once, before X is written).
+// Left: A X = alpha B.  Right: X A = alpha B.  X may alias B (B is read once,
+// before X is written).  BLAST's element recurrence (reference/Trsm.hpp:44-127)
+// subtracts the known terms in increasing k.  For left/lower and right/upper
+// those terms become known in increasing k, so a column sweep (parallel over
+// the other index) keeps the order.  For left/upper and right/lower they
+// become known in decreasing k, so each right-hand side (left) or row (right)
+// runs the recurrence as a sequential dot product in one thread.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void trsm_ref_kernel(int side_right, int upper, int unit, T alpha,
                                                              Mat<T> A, Mat<T> B, Mat<T> X) {
     int64_t const b = blockIdx.x;
     Ref<T> a(A, b), bb(B, b), x(X, b);
     int const M = X.rows, N = X.cols;
+    bool const sweep = side_right ? upper : !upper;
+    if (!sweep) {
+        if (!side_right) {
+            for (int j = threadIdx.x; j < N; j += kThreads)
+                for (int i = M - 1; i >= 0; --i) {
+                    T v = alpha * bb(i, j);
+                    for (int k = i + 1; k < M; ++k) v = __builtin_fma(-a(i, k), x(k, j), v);
+                    x(i, j) = unit ? v : v / a(i, i);
+                }
+        } else {
+            for (int i = threadIdx.x; i < M; i += kThreads)
+                for (int j = N - 1; j >= 0; --j) {
+                    T v = alpha * bb(i, j);
+                    for (int k = j + 1; k < N; ++k) v = __builtin_fma(-x(i, k), a(k, j), v);
+                    x(i, j) = unit ? v : v / a(j, j);
+                }
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < M * N; e += kThreads) {
         int const i = e % M, j = e / M;
         x(i, j) = alpha * bb(i, j);
     }
     __syncthreads();
     if (!side_right) {
-        for (int s = 0; s < M; ++s) {
-            int const k = upper ? M - 1 - s : s;
+        for (int k = 0; k < M; ++k) {
             if (!unit) {
                 T const akk = a(k, k);
                 for (int j = threadIdx.x; j < N; j += kThreads) x(k, j) = x(k, j) / akk;
                 __syncthreads();
             }
-            int const r0 = upper ? 0 : k + 1, nr = upper ? k : M - k - 1;
+            int const nr = M - k - 1;
             for (int e = threadIdx.x; e < nr * N; e += kThreads) {
-                int const i = r0 + e % nr, j = e / nr;
+                int const i = k + 1 + e % nr, j = e / nr;
                 x(i, j) = __builtin_fma(-a(i, k), x(k, j), x(i, j));
             }
             __syncthreads();
         }
     } else {
-        for (int s = 0; s < N; ++s) {
-            int const k = upper ? s : N - 1 - s;
+        for (int k = 0; k < N; ++k) {
             if (!unit) {
                 T const akk = a(k, k);
                 for (int i = threadIdx.x; i < M; i += kThreads) x(i, k) = x(i, k) / akk;
                 __syncthreads();
             }
-            int const c0 = upper ? k + 1 : 0, nc = upper ? N - k - 1 : k;
+            int const nc = N - k - 1;
             for (int e = threadIdx.x; e < M * nc; e += kThreads) {
-                int const i = e % M, j = c0 + e / M;
+                int const i = e % M, j = k + 1 + e / M;
                 x(i, j) = __builtin_fma(-x(i, k), a(k, j), x(i, j));
             }
             __syncthreads();
