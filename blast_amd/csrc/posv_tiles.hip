@@ -61,16 +61,6 @@ __device__ __forceinline__ d4 lds_to_acc(const double* tile, int lane) {
     return v;
 }
 
-__device__ __forceinline__ double readlane_d(double v, int l) {
-    long long const x = __builtin_bit_cast(long long, v);
-    int const lo = __builtin_amdgcn_readlane(int(x), l);
-    int const hi = __builtin_amdgcn_readlane(int(x >> 32), l);
-    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
-}
-
-// Tile row -> owning wavefront: rows {w, 7-w} go to wavefront w.
-__device__ __forceinline__ int owner(int i) { return i < 4 ? i : 7 - i; }
-
 // Load tile (ti, tj) of A into accumulator layout.  Outside n x n the matrix
 // is padded with the identity; the strict upper part of diagonal tiles is
 // not read (BLAST reads A's lower triangle only).
@@ -201,133 +191,128 @@ __device__ int diag_factor(double* sD, double* sW, int lane) {
     return bad;
 }
 
-template <int NT>
-struct PosvShared {
-    double P[2][NT][kTileElems];  // panel tiles of block column kb (double-buffered by kb parity)
-    double D[kTileElems];         // diagonal tile
-    double W[NT][kTileElems];     // W_kb = L_kb,kb^{-1}, kept for the solves
-    double R[NT * 16 * kMaxRhs];  // right-hand sides -> y -> x, [row][rhs]
-    double S[4][16 * kMaxRhs];    // backward-solve partial sums per wavefront
-    int info;
-};
+constexpr int tri(int i, int j) { return i * (i + 1) / 2 + j; }
 
-// Tile slots of wavefront W: row W (tiles (W, 0..W)) then row 7-W.
-template <int NT, int W>
-struct Slots {
-    static constexpr int kCount = 9;
-    static constexpr int i(int t) { return t <= W ? W : 7 - W; }
-    static constexpr int j(int t) { return t <= W ? t : t - W - 1; }
-    static constexpr bool ok(int t) { return i(t) < NT; }
-};
+// One wavefront per instance.  acc holds the lower tiles (i, j), j <= i, as
+// MFMA accumulators (AGPR/VGPR resident); LDS stages the current block
+// column, keeps W_kb = L_kb,kb^{-1} for the backward solve and holds the
+// right-hand sides.  A single wavefront needs no barriers: its LDS accesses
+// complete in program order.
+template <int NT, bool PANEL>
+__global__ __launch_bounds__(64) void posv_wave_kernel(Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X,
+                                                       int n, int nrhs, int32_t* info) {
+    __shared__ double sP[NT][kTileElems];          // block column kb: tile i at sP[i] (i >= kb)
+    __shared__ double sW[NT][kTileElems];          // W_kb
+    __shared__ double sR[NT * 16 * kMaxRhs + 64];  // RHS -> y -> x as [row][rhs]; tail: partial sums
 
-template <int NT, int W, bool PANEL>
-__device__ __forceinline__ void posv_body(PosvShared<NT>& sh, Mat<double> const& A, Mat<double> const& L,
-                                          Mat<double> const& B, Mat<double> const& X, int n, int nrhs,
-                                          int64_t b, int lane) {
-    using Sl = Slots<NT, W>;
+    int64_t const b = blockIdx.x;
+    int const lane = threadIdx.x;
     Ref<double> a(A, b), l(L, b);
     bool const lpanel = L.panel != 0;
-    d4 acc[Sl::kCount];
-    static_for<0, Sl::kCount>([&](auto T) __attribute__((always_inline)) {
-        constexpr int t = decltype(T)::value;
-        if constexpr (Sl::ok(t)) acc[t] = load_tile<PANEL>(a, n, Sl::i(t), Sl::j(t), lane);
+    constexpr int NTILES = NT * (NT + 1) / 2;
+
+    d4 acc[NTILES];
+    static_for<0, NT>([&](auto I) __attribute__((always_inline)) {
+        constexpr int i = decltype(I)::value;
+        static_for<0, i + 1>([&](auto J) __attribute__((always_inline)) {
+            constexpr int j = decltype(J)::value;
+            acc[tri(i, j)] = load_tile<PANEL>(a, n, i, j, lane);
+        });
     });
     if (nrhs > 0) {
         Ref<double> bb(B, b);
-        for (int e = threadIdx.x; e < NT * 16 * kMaxRhs; e += 256) {
+        for (int e = lane; e < NT * 16 * kMaxRhs; e += 64) {
             int const row = e >> 2, c = e & 3;
-            sh.R[e] = (row < n && c < nrhs) ? bb(row, c) : 0.0;
+            sR[e] = (row < n && c < nrhs) ? bb(row, c) : 0.0;
         }
     }
-    __syncthreads();
+    int inf = 0;
 
-    for (int kb = 0; kb < NT; ++kb) {
-        double(*P)[kTileElems] = sh.P[kb & 1];
-        // A: stage block column kb
-        static_for<0, Sl::kCount>([&](auto T) __attribute__((always_inline)) {
-            constexpr int t = decltype(T)::value;
-            if constexpr (Sl::ok(t))
-                if (Sl::j(t) == kb) acc_to_lds(Sl::i(t) == kb ? sh.D : P[Sl::i(t)], acc[t], lane);
+    static_for<0, NT>([&](auto KB) __attribute__((always_inline)) {
+        constexpr int kb = decltype(KB)::value;
+        // stage block column kb
+        static_for<0, NT>([&](auto I) __attribute__((always_inline)) {
+            constexpr int i = decltype(I)::value;
+            static_for<0, i + 1>([&](auto J) __attribute__((always_inline)) {
+                constexpr int j = decltype(J)::value;
+                if constexpr (j == kb) acc_to_lds(sP[i], acc[tri(i, j)], lane);
+            });
         });
-        __syncthreads();
-        // B: diagonal tile
-        if (W == owner(kb)) {
-            int const bad = diag_factor(sh.D, sh.W[kb], lane);
-            if (bad && lane == 0 && sh.info == 0 && 16 * kb + bad <= n) sh.info = 16 * kb + bad;
-        }
-        __syncthreads();
-        // C: panel solve L_ik = A_ik W^T; forward substitution y_kb = W b_kb
-        static_for<0, Sl::kCount>([&](auto T) __attribute__((always_inline)) {
-            constexpr int t = decltype(T)::value;
-            if constexpr (Sl::ok(t))
-                if (Sl::j(t) == kb && Sl::i(t) > kb) {
+        // diagonal tile: L_kk in sP[kb], W_kb in sW[kb]
+        int const bad = diag_factor(sP[kb], sW[kb], lane);
+        if (bad && inf == 0 && 16 * kb + bad <= n) inf = 16 * kb + bad;
+        store_tile<PANEL>(l, lpanel, n, kb, kb, lds_to_acc(sP[kb], lane), lane);
+        // panel: L_ik = A_ik W_kb^T
+        static_for<1, NT>([&](auto I) __attribute__((always_inline)) {
+            constexpr int i = decltype(I)::value;
+            static_for<0, i>([&](auto J) __attribute__((always_inline)) {
+                constexpr int j = decltype(J)::value;
+                if constexpr (j == kb) {
                     d4 z = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) z = mfma(op_read(P[Sl::i(t)], q, lane), op_read(sh.W[kb], q, lane), z);
-                    acc[t] = z;
-                    store_tile<PANEL>(l, lpanel, n, Sl::i(t), kb, z, lane);
-                    acc_to_lds(P[Sl::i(t)], z, lane);
+                    for (int q = 0; q < 4; ++q) z = mfma(op_read(sP[i], q, lane), op_read(sW[kb], q, lane), z);
+                    acc[tri(i, j)] = z;
+                    store_tile<PANEL>(l, lpanel, n, i, j, z, lane);
+                    acc_to_lds(sP[i], z, lane);
                 }
+            });
         });
-        if (W == owner(kb)) {
-            store_tile<PANEL>(l, lpanel, n, kb, kb, lds_to_acc(sh.D, lane), lane);
-            if (nrhs > 0) {
-                int const m = lane >> 2, c = lane & 3;
-                double rv[16];
-#pragma unroll
-                for (int k = 0; k < 16; ++k) rv[k] = sh.R[(16 * kb + k) * kMaxRhs + c];
-                double y = 0.0;
-#pragma unroll
-                for (int k = 0; k < 16; ++k) y = __builtin_fma(sh.W[kb][k * kTileStride + m], rv[k], y);
-                sh.R[(16 * kb + m) * kMaxRhs + c] = y;
-            }
-        }
-        __syncthreads();
-        // D: trailing update A_ij -= L_ik L_jk^T; b_i -= L_ik y_kb
-        static_for<0, Sl::kCount>([&](auto T) __attribute__((always_inline)) {
-            constexpr int t = decltype(T)::value;
-            if constexpr (Sl::ok(t) && Sl::j(t) > 0)
-                if (Sl::j(t) > kb) {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        acc[t] = mfma(-op_read(P[Sl::i(t)], q, lane), op_read(P[Sl::j(t)], q, lane), acc[t]);
-                }
-            __builtin_amdgcn_sched_barrier(0);
-        });
+        // forward substitution: y_kb = W_kb b_kb; b_i -= L_ik y_kb
         if (nrhs > 0) {
             int const m = lane >> 2, c = lane & 3;
-            static_for<0, 2>([&](auto R) __attribute__((always_inline)) {
-                constexpr int i = decltype(R)::value == 0 ? W : 7 - W;
-                if constexpr (i < NT)
-                    if (i > kb) {
-                        double v = sh.R[(16 * i + m) * kMaxRhs + c];
+            double rv[16];
 #pragma unroll
-                        for (int k = 0; k < 16; ++k)
-                            v = __builtin_fma(-P[i][k * kTileStride + m], sh.R[(16 * kb + k) * kMaxRhs + c], v);
-                        sh.R[(16 * i + m) * kMaxRhs + c] = v;
-                    }
+            for (int k = 0; k < 16; ++k) rv[k] = sR[(16 * kb + k) * kMaxRhs + c];
+            double y = 0.0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) y = __builtin_fma(sW[kb][k * kTileStride + m], rv[k], y);
+            sR[(16 * kb + m) * kMaxRhs + c] = y;
+            static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
+                constexpr int i = decltype(I)::value;
+                double v = sR[(16 * i + m) * kMaxRhs + c];
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                    v = __builtin_fma(-sP[i][k * kTileStride + m], sR[(16 * kb + k) * kMaxRhs + c], v);
+                sR[(16 * i + m) * kMaxRhs + c] = v;
+                __builtin_amdgcn_sched_barrier(0);
             });
         }
-    }
+        // trailing update: A_ij -= L_ik L_jk^T for kb < j <= i, operands
+        // loaded once per block column
+        double f[NT][4];
+        static_for<1, NT>([&](auto I) __attribute__((always_inline)) {
+            constexpr int i = decltype(I)::value;
+            if constexpr (i > kb)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) f[i][q] = op_read(sP[i], q, lane);
+        });
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            static_for<1, NT>([&](auto I) __attribute__((always_inline)) {
+                constexpr int i = decltype(I)::value;
+                static_for<1, i + 1>([&](auto J) __attribute__((always_inline)) {
+                    constexpr int j = decltype(J)::value;
+                    if constexpr (j > kb) acc[tri(i, j)] = mfma(-f[i][q], f[j][q], acc[tri(i, j)]);
+                });
+            });
+    });
 
     if (nrhs > 0) {
-        __syncthreads();
-        // Backward substitution x = L^{-T} y, block column kb = NT-1..0, with
-        // L_ik taken from the accumulators.
-        for (int kb = NT - 1; kb >= 0; --kb) {
+        // backward substitution x = L^{-T} y with L_ik from the accumulators
+        double* sSum = sR + NT * 16 * kMaxRhs;
+        static_for<0, NT>([&](auto KK) __attribute__((always_inline)) {
+            constexpr int kb = NT - 1 - decltype(KK)::value;
             double p[kMaxRhs] = {0.0, 0.0, 0.0, 0.0};
-            static_for<0, Sl::kCount>([&](auto T) __attribute__((always_inline)) {
-                constexpr int t = decltype(T)::value;
-                if constexpr (Sl::ok(t) && Sl::i(t) > Sl::j(t))
-                    if (Sl::j(t) == kb) {
+            static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
+                constexpr int i = decltype(I)::value;
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            int const row = 16 * Sl::i(t) + acc_row(lane, r);
+                for (int r = 0; r < 4; ++r) {
+                    int const row = 16 * i + acc_row(lane, r);
 #pragma unroll
-                            for (int c = 0; c < kMaxRhs; ++c)
-                                p[c] = __builtin_fma(acc[t][r], sh.R[row * kMaxRhs + c], p[c]);
-                        }
-                    }
+                    for (int c = 0; c < kMaxRhs; ++c)
+                        p[c] = __builtin_fma(acc[tri(i, kb)][r], sR[row * kMaxRhs + c], p[c]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
             });
 #pragma unroll
             for (int c = 0; c < kMaxRhs; ++c) {
@@ -336,47 +321,23 @@ __device__ __forceinline__ void posv_body(PosvShared<NT>& sh, Mat<double> const&
             }
             if (lane < 16)
 #pragma unroll
-                for (int c = 0; c < kMaxRhs; ++c) sh.S[W][lane * kMaxRhs + c] = p[c];
-            __syncthreads();
-            if (W == owner(kb)) {
-                int const m = lane >> 2, c = lane & 3;
-                double tv[16];
+                for (int c = 0; c < kMaxRhs; ++c) sSum[lane * kMaxRhs + c] = p[c];
+            int const m = lane >> 2, c = lane & 3;
+            double tv[16];
 #pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    int const e = k * kMaxRhs + c;
-                    tv[k] = sh.R[(16 * kb + k) * kMaxRhs + c] - ((sh.S[0][e] + sh.S[1][e]) + (sh.S[2][e] + sh.S[3][e]));
-                }
-                double x = 0.0;
+            for (int k = 0; k < 16; ++k) tv[k] = sR[(16 * kb + k) * kMaxRhs + c] - sSum[k * kMaxRhs + c];
+            double x = 0.0;
 #pragma unroll
-                for (int k = 0; k < 16; ++k) x = __builtin_fma(sh.W[kb][m * kTileStride + k], tv[k], x);
-                sh.R[(16 * kb + m) * kMaxRhs + c] = x;
-            }
-            __syncthreads();
-        }
+            for (int k = 0; k < 16; ++k) x = __builtin_fma(sW[kb][m * kTileStride + k], tv[k], x);
+            sR[(16 * kb + m) * kMaxRhs + c] = x;
+        });
         Ref<double> xx(X, b);
-        for (int e = threadIdx.x; e < n * nrhs; e += 256) {
+        for (int e = lane; e < n * nrhs; e += 64) {
             int const row = e % n, c = e / n;
-            xx(row, c) = sh.R[row * kMaxRhs + c];
+            xx(row, c) = sR[row * kMaxRhs + c];
         }
     }
-}
-
-template <int NT, bool PANEL>
-__global__ __launch_bounds__(256) void posv_tile_kernel(Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X,
-                                                        int n, int nrhs, int32_t* info) {
-    __shared__ PosvShared<NT> sh;
-    int64_t const b = blockIdx.x;
-    int const lane = threadIdx.x & 63;
-    int const w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    if (threadIdx.x == 0) sh.info = 0;
-    switch (w) {
-        case 0: posv_body<NT, 0, PANEL>(sh, A, L, B, X, n, nrhs, b, lane); break;
-        case 1: posv_body<NT, 1, PANEL>(sh, A, L, B, X, n, nrhs, b, lane); break;
-        case 2: posv_body<NT, 2, PANEL>(sh, A, L, B, X, n, nrhs, b, lane); break;
-        default: posv_body<NT, 3, PANEL>(sh, A, L, B, X, n, nrhs, b, lane); break;
-    }
-    __syncthreads();
-    if (info && threadIdx.x == 0) info[b] = sh.info;
+    if (info && lane == 0) info[b] = inf;
 }
 
 template <int NT>
@@ -384,12 +345,12 @@ static int launch_nt(hipStream_t s, Mat<double> A, Mat<double> L, Mat<double> B,
                      int64_t batch, int32_t* info) {
     bool const panel = A.panel && !A.trans && L.panel && !L.trans;
     if (panel)
-        hipLaunchKernelGGL((posv_tile_kernel<NT, true>), dim3(unsigned(batch)), dim3(256), 0, s, A, L, B, X, n, nrhs,
+        hipLaunchKernelGGL((posv_wave_kernel<NT, true>), dim3(unsigned(batch)), dim3(64), 0, s, A, L, B, X, n, nrhs,
                            info);
     else
-        hipLaunchKernelGGL((posv_tile_kernel<NT, false>), dim3(unsigned(batch)), dim3(256), 0, s, A, L, B, X, n,
+        hipLaunchKernelGGL((posv_wave_kernel<NT, false>), dim3(unsigned(batch)), dim3(64), 0, s, A, L, B, X, n,
                            nrhs, info);
-    return check_launch("posv_tile_kernel");
+    return check_launch("posv_wave_kernel");
 }
 
 // Fused potrf (+ forward/backward solve when nrhs > 0) for n <= 128,
