@@ -45,6 +45,8 @@ def parse():
     p.add_argument("--batch", type=int, default=BATCH)
     p.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="first run a 256 MiB device copy (known HBM bytes) to scale rocprofv3 FETCH/WRITE_SIZE")
     return p.parse_args()
 
 
@@ -96,6 +98,22 @@ def backward_errors(la, lb, abuf, xbuf, bbuf, batch, lbuf=None):
     se = torch.cat(out_solve).numpy()
     fe = torch.cat(out_fact).numpy() if out_fact else None
     return se, fe
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", "posv_wave_n128_hbm_traffic.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    if kernel not in text:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", text)
+    return int(m.group(1)) if m else None
 
 
 def time_steps(fn, steps, warmup):
@@ -152,6 +170,12 @@ def main():
     from blast_amd import capi
 
     batch = args.batch
+    if args.calibrate_copy:
+        src = torch.ones(256 * 2 ** 20 // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     la, lb, abuf, bbuf = make_inputs(batch)
     lbuf = torch.zeros_like(abuf)
     xbuf = torch.zeros_like(bbuf)
@@ -174,10 +198,38 @@ def main():
             "tflops": FLOPS_PER_INSTANCE * batch / t_plain / 1e12,
             "max_solve_backward_error": float(se.max()),
             "max_potrf_backward_error": float(fe.max()),
+            "contract": "dposv: A (lower) read, L and X written",
         }
     }
-    value, t_step, dtype = plain_rate, t_plain, "f64"
-    headline = "plain_fp64"
+    # Mixed precision + FP64 refinement (dsposv contract: A read, X written;
+    # A changes only for instances that fall back to FP64, none here).
+    it = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    for name, prec in (("mixed_fp32_ir", 0), ("mixed_bf16_ir", 1)):
+        xm = torch.zeros_like(bbuf)
+        Xm = capi.Operand(lb, xm)
+
+        def mixed():
+            capi.sposv(prec, A, B, Xm, batch, it, info)
+
+        t_m = time_steps(mixed, args.steps, args.warmup)
+        sem, _ = backward_errors(la, lb, abuf, xm, bbuf, batch)
+        its = it.cpu().numpy()
+        result[name] = {
+            "solves_per_s": batch / t_m,
+            "ms_per_step": t_m * 1e3,
+            "speedup_vs_plain_fp64": t_plain / t_m,
+            "max_solve_backward_error": float(sem.max()),
+            "refinement_steps": {str(k): int((its == k).sum()) for k in sorted(set(its.tolist()))},
+            "fell_back_to_fp64": int((its < 0).sum()),
+            "contract": "dsposv: A read, X written",
+        }
+    mixed = result["mixed_fp32_ir"]
+    if mixed["max_solve_backward_error"] <= 1e-12:
+        value, t_step, dtype = mixed["solves_per_s"], mixed["ms_per_step"] / 1e3, "f32 factor + f64 refinement"
+        headline = "mixed_fp32_ir"
+    else:
+        value, t_step, dtype = plain_rate, t_plain, "f64"
+        headline = "plain_fp64"
 
     achieved = BYTES_PLAIN * batch / t_plain / 1e9
     line = {
@@ -205,8 +257,8 @@ def main():
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
-            "traffic": None,
-            "kernel": "posv_tile_kernel<8>",
+            "traffic": profiled_traffic("posv_wave_kernel<8, true>"),
+            "kernel": "posv_wave_kernel<8, true>",
             "algorithmic_bytes_per_instance": BYTES_PLAIN,
             "fp64_tflops": FLOPS_PER_INSTANCE * batch / t_plain / 1e12,
             "fp64_frac_of_peak": FLOPS_PER_INSTANCE * batch / t_plain / 1e12 / PEAK_FP64_TFLOPS,

@@ -1,0 +1,207 @@
+// MFMA tile helpers shared by the fused Cholesky kernels (FP64 and FP32).
+//
+// Tiles are 16x16.  Register tiles use the accumulator layout of
+// v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32; LDS tiles store T[m][k] at
+// k*17 + m (the 17 avoids bank conflicts for both the accumulator writes and
+// the operand reads).
+#pragma once
+
+#include <type_traits>
+
+#include "common.hpp"
+
+namespace blast_hip {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTileStride = 17;
+constexpr int kTileElems = 16 * kTileStride;
+constexpr int kMaxRhs = 4;
+
+constexpr int tri(int i, int j) { return i * (i + 1) / 2 + j; }
+
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// Operand layouts (both precisions, lane l): A operand A[l % 16][l / 16],
+// B operand B[l / 16][l % 16] for the K-chunk.  Accumulator register r of
+// lane l holds D[row][l % 16]:
+//   f64 16x16x4: row = l / 16 + 4 r
+//   f32 16x16x4: row = 4 (l / 16) + r
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    using acc_t = d4;
+    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float> {
+    using acc_t = f4;
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+__device__ __forceinline__ int acc_col(int lane) { return lane & 15; }
+
+// Operand element for K-chunk q: T[l % 16][4q + l / 16] of an LDS tile.
+template <typename T>
+__device__ __forceinline__ T op_read(const T* tile, int q, int lane) {
+    return tile[(4 * q + (lane >> 4)) * kTileStride + (lane & 15)];
+}
+
+template <typename T>
+__device__ __forceinline__ void acc_to_lds(T* tile, typename Mfma<T>::acc_t v, int lane) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tile[acc_col(lane) * kTileStride + Mfma<T>::row(lane, r)] = v[r];
+}
+
+template <typename T>
+__device__ __forceinline__ typename Mfma<T>::acc_t lds_to_acc(const T* tile, int lane) {
+    typename Mfma<T>::acc_t v;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = tile[acc_col(lane) * kTileStride + Mfma<T>::row(lane, r)];
+    return v;
+}
+
+// Element access; PANEL = column-major panel storage, not transposed (the
+// layout of BLAST's StaticPanelMatrix), which needs no runtime dispatch.
+template <bool PANEL>
+__device__ __forceinline__ double& at(Ref<double> const& a, int row, int col) {
+    if constexpr (PANEL)
+        return a.p[int64_t(row >> 2) * a.m.spacing + (row & 3) + 4 * col];
+    else
+        return a(row, col);
+}
+
+// Tile (ti, tj) of the FP64 matrix A in accumulator layout of precision T.
+// Outside n x n the matrix is padded with the identity; the strict upper
+// part of diagonal tiles is not read (BLAST reads A's lower triangle only).
+template <bool PANEL, typename T>
+__device__ __forceinline__ typename Mfma<T>::acc_t load_tile(Ref<double> const& a, int n, int ti, int tj, int lane) {
+    typename Mfma<T>::acc_t v;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        int const row = 16 * ti + Mfma<T>::row(lane, r), col = 16 * tj + acc_col(lane);
+        v[r] = T((row < n && col < n) ? (row >= col ? at<PANEL>(a, row, col) : 0.0) : (row == col ? 1.0 : 0.0));
+    }
+    return v;
+}
+
+// Store FP64 tile (ti, tj) of L.  Dense storage: i >= j only.  Panel storage:
+// also zeros in the strict upper part of each 4x4 diagonal block (what
+// BLAST's panel potrf leaves there is rounding residue of 0).
+template <bool PANEL>
+__device__ __forceinline__ void store_tile(Ref<double> const& l, bool panel, int n, int ti, int tj, d4 v, int lane) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        int const row = 16 * ti + Mfma<double>::row(lane, r), col = 16 * tj + acc_col(lane);
+        if (row < n && col < n && (row >= col || (panel && (row >> 2) == (col >> 2))))
+            at<PANEL>(l, row, col) = row >= col ? v[r] : 0.0;
+    }
+}
+
+// Broadcast lane C of every 16-lane row to the whole row (DPP row_newbcast).
+template <int C>
+__device__ __forceinline__ double row_bcast(double v) {
+    long long const x = __builtin_bit_cast(long long, v);
+    int const lo = __builtin_amdgcn_mov_dpp(int(x), 0x150 + C, 0xF, 0xF, false);
+    int const hi = __builtin_amdgcn_mov_dpp(int(x >> 32), 0x150 + C, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+template <int C>
+__device__ __forceinline__ float row_bcast(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x150 + C, 0xF, 0xF, false));
+}
+
+// 1/sqrt(d): hardware estimate refined by Newton steps (about 1 ulp).
+__device__ __forceinline__ double rsqrt_nr(double d) {
+    double r = __builtin_amdgcn_rsq(d);
+    double const h = 0.5 * d;
+    r = __builtin_fma(r, __builtin_fma(-h * r, r, 0.5), r);
+    r = __builtin_fma(r, __builtin_fma(-h * r, r, 0.5), r);
+    return r;
+}
+__device__ __forceinline__ float rsqrt_nr(float d) {
+    float r = __builtin_amdgcn_rsqf(d);
+    float const h = 0.5f * d;
+    return __builtin_fmaf(r, __builtin_fmaf(-h * r, r, 0.5f), r);
+}
+
+// Factorisation of the 16x16 diagonal tile in sD, and its inverse.
+// Each 16-lane row of the wavefront holds a copy of the tile, lane i of the
+// row holding tile row i; every broadcast is a DPP row_newbcast.
+// 1. Cholesky, left-looking: for column k, lane i forms a_ik - <l_i, l_k>
+//    (row k broadcast from lane k), then r = 1/sqrt(d), l_kk = d*r,
+//    l_ik = (.)*r.
+// 2. W = L_kk^{-1}, lane i computing row i: row j is finalised at step j
+//    and broadcast from lane j to the lanes below it.
+// On exit sD holds L_kk (zeros above the diagonal) and sW holds W, both as
+// T[m][k] at k*17 + m.  Returns 0 or the 1-based tile column of the first
+// non-positive pivot.  Branch-free: every lane runs the same instructions.
+template <typename T>
+__device__ int diag_factor(T* sD, T* sW, int lane) {
+    int const i = lane & 15;
+    T t[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) t[c] = sD[c * kTileStride + i];
+    int bad = 0;
+    T myr = T(0);
+    static_for<0, 16>([&](auto K) __attribute__((always_inline)) {
+        constexpr int k = decltype(K)::value;
+        T acc0 = t[k], acc1 = T(0);
+        static_for<0, k>([&](auto J) __attribute__((always_inline)) {
+            constexpr int j = decltype(J)::value;
+            if constexpr (j % 2 == 0)
+                acc0 = __builtin_fma(-t[j], row_bcast<k>(t[j]), acc0);
+            else
+                acc1 = __builtin_fma(-t[j], row_bcast<k>(t[j]), acc1);
+        });
+        T const v = acc0 + acc1;
+        T const d = row_bcast<k>(v);
+        bad = (bad == 0 && !(d > T(0))) ? k + 1 : bad;
+        T const r = rsqrt_nr(d);
+        myr = (i == k) ? r : myr;
+        T lik = ((i == k) ? d : v) * r;
+        lik = (i < k) ? T(0) : lik;
+        t[k] = lik;
+        __builtin_amdgcn_sched_barrier(0);
+    });
+    // W row by row: at step j lane j finalises W[j][:] = (e_j - acc) / l_jj;
+    // lanes i > j then accumulate acc[c] += l_ij W[j][c], c <= j.
+    T w[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) w[c] = T(0);
+    static_for<0, 16>([&](auto J) __attribute__((always_inline)) {
+        constexpr int j = decltype(J)::value;
+        static_for<0, j>([&](auto C) __attribute__((always_inline)) {
+            constexpr int c = decltype(C)::value;
+            w[c] = (i == j) ? -w[c] * myr : w[c];
+        });
+        w[j] = (i == j) ? myr : w[j];
+        T const f = (i > j) ? t[j] : T(0);
+        static_for<0, j + 1>([&](auto C) __attribute__((always_inline)) {
+            constexpr int c = decltype(C)::value;
+            w[c] = __builtin_fma(f, row_bcast<j>(w[c]), w[c]);
+        });
+        __builtin_amdgcn_sched_barrier(0);
+    });
+    if (lane < 16) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            sD[c * kTileStride + i] = (c <= i) ? t[c] : T(0);  // L[i][c]
+            sW[c * kTileStride + i] = (c <= i) ? w[c] : T(0);  // W[i][c]
+        }
+    }
+    return bad;
+}
+
+}  // namespace blast_hip

@@ -79,3 +79,72 @@ def test_posv_alias_x_b():
     A, B = dev(la, la.pack(a)), dev(lb, lb.pack(b))
     capi.posv(A, A, B, B, batch)  # L aliases A, X aliases B
     assert np.all(solve_backward_error(a, B.logical(batch), b) < 4 * n * EPS64)
+
+
+# ------------------------------------------------------------------ mixed precision + FP64 refinement
+def _haar(rng, n):
+    q, r = np.linalg.qr(rng.standard_normal((n, n)))
+    return q * np.sign(np.diag(r))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("n", [1, 7, 16, 33, 64, 128])
+@pytest.mark.parametrize("nrhs", [1, 4])
+def test_sposv_ir_converges_to_fp64(precision, n, nrhs):
+    import torch
+
+    rng = np.random.default_rng(100 * n + nrhs + precision)
+    batch = 9
+    a = spd(rng, batch, n)
+    b = rng.uniform(-1, 1, (batch, n, nrhs))
+    la, lb = Layout.make(n, n), Layout.make(n, nrhs)
+    ah = la.pack(a)
+    A, B, X = dev(la, ah), dev(lb, lb.pack(b)), capi.Operand.empty(lb, batch)
+    it = torch.full((batch,), -9, dtype=torch.int32, device="cuda")
+    info = torch.full((batch,), -9, dtype=torch.int32, device="cuda")
+    capi.sposv(precision, A, B, X, batch, it, info)
+    its, inf = it.cpu().numpy(), info.cpu().numpy()
+    assert np.all(its >= 0) and np.all(inf == 0), (its, inf)
+    assert np.all(its <= (3 if precision == 0 else 30)), its
+    x = X.logical(batch)
+    assert np.all(solve_backward_error(a, x, b) < 1e-15)
+    np.testing.assert_allclose(x, np.linalg.solve(a, b), rtol=0, atol=1e-12 * np.abs(x).max())
+    np.testing.assert_array_equal(A.host(), ah)  # A untouched when refinement converges
+
+
+def test_sposv_ir_falls_back_for_hard_instances():
+    import torch
+
+    rng = np.random.default_rng(7)
+    n, batch = 64, 4
+    a = spd(rng, batch, n)
+    u = _haar(rng, n)
+    a[1] = (u * np.logspace(0, -13, n)) @ u.T  # SPD, cond 1e13: beyond FP32
+    a[1] = 0.5 * (a[1] + a[1].T)
+    b = rng.uniform(-1, 1, (batch, n, 2))
+    la, lb = Layout.make(n, n), Layout.make(n, 2)
+    A, B, X = dev(la, la.pack(a)), dev(lb, lb.pack(b)), capi.Operand.empty(lb, batch)
+    it = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    info = torch.full((batch,), -9, dtype=torch.int32, device="cuda")
+    capi.sposv(0, A, B, X, batch, it, info)
+    its = it.cpu().numpy()
+    assert its[1] < 0 and np.all(its[[0, 2, 3]] >= 0), its
+    x = X.logical(batch)
+    assert np.all(solve_backward_error(a, x, b) < 1e-14)
+    lfac = A.logical(batch)[1]  # the fallback wrote the FP64 factor into A
+    assert potrf_backward_error(a[1], lfac) < 1e-14
+
+
+def test_sposv_uncovered_shape_uses_fp64():
+    import torch
+
+    rng = np.random.default_rng(8)
+    n, batch = 150, 3
+    a = spd(rng, batch, n)
+    b = rng.uniform(-1, 1, (batch, n, 5))
+    la, lb = Layout.make(n, n), Layout.make(n, 5)
+    A, B, X = dev(la, la.pack(a)), dev(lb, lb.pack(b)), capi.Operand.empty(lb, batch)
+    it = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    capi.sposv(0, A, B, X, batch, it)
+    assert np.all(it.cpu().numpy() == -3)
+    assert np.all(solve_backward_error(a, X.logical(batch), b) < 1e-15)
