@@ -204,12 +204,15 @@ def main():
     # Mixed precision + FP64 refinement (dsposv contract: A read, X written;
     # A changes only for instances that fall back to FP64, none here).
     it = torch.zeros(batch, dtype=torch.int32, device="cuda")
-    for name, prec in (("mixed_fp32_ir", 0), ("mixed_bf16_ir", 1)):
+    # tol: stop when ||b - A x||_inf <= tol ||A||_inf ||x||_inf (the 1e-12
+    # backward-error bar with margin); tol = 0 is LAPACK dsposv's eps*sqrt(n).
+    for name, prec, tol in (("mixed_fp32_ir", 0, 1e-13), ("mixed_bf16_ir", 1, 1e-13),
+                            ("mixed_fp32_ir_lapack_test", 0, 0.0)):
         xm = torch.zeros_like(bbuf)
         Xm = capi.Operand(lb, xm)
 
         def mixed():
-            capi.sposv(prec, A, B, Xm, batch, it, info)
+            capi.sposv(prec, A, B, Xm, batch, it, info, tol=tol)
 
         t_m = time_steps(mixed, args.steps, args.warmup)
         sem, _ = backward_errors(la, lb, abuf, xm, bbuf, batch)
@@ -221,6 +224,7 @@ def main():
             "max_solve_backward_error": float(sem.max()),
             "refinement_steps": {str(k): int((its == k).sum()) for k in sorted(set(its.tolist()))},
             "fell_back_to_fp64": int((its < 0).sum()),
+            "stopping_tol": tol if tol > 0 else "eps*sqrt(n)",
             "contract": "dsposv: A read, X written",
         }
     mixed = result["mixed_fp32_ir"]

@@ -64,7 +64,7 @@ def lib():
             getattr(L, f"blast_hip_{p}potrf_batched").argtypes = [vp, P, P, i64, i32p]
             getattr(L, f"blast_hip_{p}getrf_batched").argtypes = [vp, P, vp, i64, i64, i32p]
         L.blast_hip_dposv_batched.argtypes = [vp, P, P, P, P, i64, i32p]
-        L.blast_hip_dsposv_batched.argtypes = [vp, c_int, c_int, P, P, P, i64, i32p, i32p]
+        L.blast_hip_dsposv_batched.argtypes = [vp, c_int, c_int, ctypes.c_double, P, P, P, i64, i32p, i32p]
         L.blast_hip_set_algorithm.argtypes = [c_int]
         L.blast_hip_last_error.restype = ctypes.c_char_p
         L.blast_hip_version.restype = ctypes.c_char_p
@@ -186,8 +186,8 @@ def posv(A, L, B, X, batch, info=None, stream=None):
                                                      _info_ptr(info)))
 
 
-def sposv(precision, A, B, X, batch, iter_, info=None, itmax=30, stream=None):
+def sposv(precision, A, B, X, batch, iter_, info=None, itmax=30, tol=0.0, stream=None):
     """Mixed precision (0 = FP32, 1 = BF16 products) + FP64 refinement."""
     name = "blast_hip_dsposv_batched"
-    return check(name, lib().blast_hip_dsposv_batched(_stream(stream), int(precision), int(itmax), _ref(A), _ref(B),
-                                                      _ref(X), batch, _info_ptr(info), _info_ptr(iter_)))
+    return check(name, lib().blast_hip_dsposv_batched(_stream(stream), int(precision), int(itmax), float(tol), _ref(A),
+                                                      _ref(B), _ref(X), batch, _info_ptr(info), _info_ptr(iter_)))

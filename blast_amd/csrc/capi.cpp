@@ -171,20 +171,21 @@ int posv_entry(void* stream, const blast_matrix_t* A, const blast_matrix_t* L, c
                             Mat<T>::from(*X), batch, info);
 }
 
-int sposv_entry(void* stream, int precision, int itmax, const blast_matrix_t* A, const blast_matrix_t* B,
-                const blast_matrix_t* X, int64_t batch, int32_t* info, int32_t* iter) {
+int sposv_entry(void* stream, int precision, int itmax, double tol, const blast_matrix_t* A,
+                const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch, int32_t* info, int32_t* iter) {
     if (precision != 0 && precision != 1) { set_error("dsposv: precision must be 0 (FP32) or 1 (BF16)"); return -2; }
     if (itmax < 0 || itmax > 1000) { set_error("dsposv: itmax out of range"); return -3; }
-    if (!valid<double>(A, "A")) return -4;
-    if (!valid<double>(B, "B")) return -5;
-    if (!valid<double>(X, "X")) return -6;
-    if (!valid_batch(batch)) return -7;
-    if (batch > 0 && !iter) { set_error("dsposv: iter is required"); return -9; }
-    if (A->rows != A->cols) { set_error("dsposv: A must be square"); return -4; }
-    if (B->rows != A->rows) { set_error("dsposv: Invalid argument size (B)"); return -5; }
-    if (X->rows != B->rows || X->cols != B->cols) { set_error("dsposv: Invalid argument size (X)"); return -6; }
+    if (!(tol >= 0.0 && tol < 1.0)) { set_error("dsposv: tol must be in [0, 1)"); return -4; }
+    if (!valid<double>(A, "A")) return -5;
+    if (!valid<double>(B, "B")) return -6;
+    if (!valid<double>(X, "X")) return -7;
+    if (!valid_batch(batch)) return -8;
+    if (batch > 0 && !iter) { set_error("dsposv: iter is required"); return -10; }
+    if (A->rows != A->cols) { set_error("dsposv: A must be square"); return -5; }
+    if (B->rows != A->rows) { set_error("dsposv: Invalid argument size (B)"); return -6; }
+    if (X->rows != B->rows || X->cols != B->cols) { set_error("dsposv: Invalid argument size (X)"); return -7; }
     (void)hipGetLastError();  // drop a stale error left by an earlier, unrelated HIP call
-    return dispatch_dsposv(hipStream_t(stream), precision, itmax, Mat<double>::from(*A), Mat<double>::from(*B),
+    return dispatch_dsposv(hipStream_t(stream), precision, itmax, tol, Mat<double>::from(*A), Mat<double>::from(*B),
                            Mat<double>::from(*X), batch, info, iter);
 }
 
@@ -257,9 +258,10 @@ int blast_hip_dposv_batched(void* s, const blast_matrix_t* A, const blast_matrix
     return posv_entry<double>(s, A, L, B, X, batch, info);
 }
 
-int blast_hip_dsposv_batched(void* s, int precision, int itmax, const blast_matrix_t* A, const blast_matrix_t* B,
-                             const blast_matrix_t* X, int64_t batch, int32_t* info, int32_t* iter) {
-    return sposv_entry(s, precision, itmax, A, B, X, batch, info, iter);
+int blast_hip_dsposv_batched(void* s, int precision, int itmax, double tol, const blast_matrix_t* A,
+                             const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch, int32_t* info,
+                             int32_t* iter) {
+    return sposv_entry(s, precision, itmax, tol, A, B, X, batch, info, iter);
 }
 
 }  // extern "C"

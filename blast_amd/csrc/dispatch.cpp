@@ -63,9 +63,9 @@ int dispatch_posv(hipStream_t s, Mat<T> A, Mat<T> L, Mat<T> B, Mat<T> X, int64_t
 // Mixed precision + FP64 refinement; instances that do not converge (or
 // shapes the mixed kernel does not cover) are solved by the FP64 path with
 // the factor written into A, as LAPACK dsposv does.
-int dispatch_dsposv(hipStream_t s, int bf16, int itmax, Mat<double> A, Mat<double> B, Mat<double> X, int64_t batch,
-                    int32_t* info, int32_t* iter) {
-    int r = launch_sposv_ir(s, bf16, A, B, X, itmax, batch, info, iter);
+int dispatch_dsposv(hipStream_t s, int bf16, int itmax, double tol, Mat<double> A, Mat<double> B, Mat<double> X,
+                    int64_t batch, int32_t* info, int32_t* iter) {
+    int r = launch_sposv_ir(s, bf16, A, B, X, itmax, tol, batch, info, iter);
     if (r == 0) return launch_posv_tiles(s, A, A, B, X, batch, info, iter);
     if (r != 1) return r;
     r = launch_fill_i32(s, iter, -3, batch);

@@ -21,11 +21,11 @@ template <typename T> int launch_getrf_ref(hipStream_t, Mat<T>, int64_t*, int64_
 int launch_posv_tiles(hipStream_t, Mat<double>, Mat<double>, Mat<double>, Mat<double>, int64_t, int32_t*,
                       const int32_t* only_if = nullptr);
 // sposv_ir.hip: returns 1 when the shape is not covered
-int launch_sposv_ir(hipStream_t, int bf16, Mat<double> A, Mat<double> B, Mat<double> X, int itmax, int64_t batch,
-                    int32_t* info, int32_t* iter);
+int launch_sposv_ir(hipStream_t, int bf16, Mat<double> A, Mat<double> B, Mat<double> X, int itmax, double tol,
+                    int64_t batch, int32_t* info, int32_t* iter);
 int launch_fill_i32(hipStream_t, int32_t* p, int32_t v, int64_t count);
-int dispatch_dsposv(hipStream_t, int bf16, int itmax, Mat<double> A, Mat<double> B, Mat<double> X, int64_t batch,
-                    int32_t* info, int32_t* iter);
+int dispatch_dsposv(hipStream_t, int bf16, int itmax, double tol, Mat<double> A, Mat<double> B, Mat<double> X,
+                    int64_t batch, int32_t* info, int32_t* iter);
 
 // dispatch.cpp
 template <typename T> int dispatch_gemm(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t);

@@ -11,7 +11,8 @@
 //   2. x = 0, r = b; repeat: d = L^{-T} L^{-1} r in FP32, x += d in FP64,
 //      r = b - A x in FP64 with A read again from HBM (L2/MALL resident),
 //      until max|r| <= max|x| * ||A||_inf * eps * sqrt(n) per right-hand side
-//      (LAPACK dsposv's test, BWDMAX = 1) or itmax refinements.
+//      (LAPACK dsposv's test, BWDMAX = 1; or tol instead of eps*sqrt(n)
+//      when tol > 0) or itmax refinements.
 // iter[b] = number of refinement steps (0 = the first solve converged),
 // -1 = the FP32 factorisation broke down, -2 = no convergence; negative
 // instances are then solved by the FP64 kernel (posv_tiles.hip) in place.
@@ -45,7 +46,7 @@ __device__ __forceinline__ double a_sym(Ref<double> const& a, int row, int col) 
 
 template <int NT, bool PANEL, bool BF16>
 __global__ __launch_bounds__(64) void sposv_ir_kernel(Mat<double> A, Mat<double> B, Mat<double> X, int n, int nrhs,
-                                                      int itmax, int32_t* info, int32_t* iter) {
+                                                      int itmax, double tol, int32_t* info, int32_t* iter) {
     constexpr int NTILES = NT * (NT + 1) / 2;
     constexpr int NV = NT * 16 * kMaxRhs;
     __shared__ float sL[NTILES][kTileElems];  // FP32 factor; tile (k,k) holds W_k = L_kk^{-1}
@@ -128,73 +129,81 @@ __global__ __launch_bounds__(64) void sposv_ir_kernel(Mat<double> A, Mat<double>
     }
 
     // ---------------------------------------------------------- refine
-    // Residual lanes: lane = row within a 64-row half; solve lanes: (m, c).
+    // Solve lanes: (m, c) = (row within a 16-row block, right-hand side).
+    // Residual lanes: one row each, 64 rows per pass.
     int const m = lane >> 2, c = lane & 3;
     for (int e = lane; e < NV; e += 64) {
         int const row = e >> 2, cc = e & 3;
         sX[e] = 0.0;
         sV[e] = (row < n && cc < nrhs) ? float(bb(row, cc)) : 0.f;
     }
-    // ||A||_inf (symmetric: max absolute row sum) and max |b| per column
-    double anrm = 0.0;
-    for (int h = 0; h < NT * 16; h += 64) {
-        int const row = h + lane;
-        double s = 0.0;
-        if (row < n)
-            for (int col = 0; col < n; ++col) s += __builtin_fabs(a_sym<PANEL>(a, row, col));
-        anrm = __builtin_fmax(anrm, s);
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) anrm = __builtin_fmax(anrm, __shfl_xor(anrm, o));
-    double const cte = anrm * 1.1102230246251565e-16 * __builtin_sqrt(double(n));  // eps = 2^-53
-
+    double anrm = 0.0;  // ||A||_inf, accumulated during the first residual
     int it = 0;
     bool converged = false;
     for (;; ++it) {
-        // d = L^{-T} L^{-1} v (FP32), forward then backward by 16-row blocks
-        for (int kb = 0; kb < NT; ++kb) {
+        // d = L^{-T} L^{-1} v in FP32, by 16-row blocks
+        static_for<0, NT>([&](auto KB) __attribute__((always_inline)) {
+            constexpr int kb = decltype(KB)::value;
             float t = sV[(16 * kb + m) * kMaxRhs + c];
-            for (int j = 0; j < kb; ++j) {
-                const float* Lt = sL[kb * (kb + 1) / 2 + j];
+            static_for<0, kb>([&](auto J) __attribute__((always_inline)) {
+                constexpr int j = decltype(J)::value;
+                const float* Lt = sL[tri(kb, j)];
 #pragma unroll
                 for (int k = 0; k < 16; ++k) t = __builtin_fmaf(-Lt[k * kTileStride + m], sV[(16 * j + k) * kMaxRhs + c], t);
-            }
+            });
             sT[m * kMaxRhs + c] = t;
-            const float* W = sL[kb * (kb + 1) / 2 + kb];
+            const float* W = sL[tri(kb, kb)];
             float y = 0.f;
 #pragma unroll
             for (int k = 0; k < 16; ++k) y = __builtin_fmaf(W[k * kTileStride + m], sT[k * kMaxRhs + c], y);
             sV[(16 * kb + m) * kMaxRhs + c] = y;
-        }
-        for (int kb = NT - 1; kb >= 0; --kb) {
+        });
+        static_for<0, NT>([&](auto KK) __attribute__((always_inline)) {
+            constexpr int kb = NT - 1 - decltype(KK)::value;
             float t = sV[(16 * kb + m) * kMaxRhs + c];
-            for (int i = kb + 1; i < NT; ++i) {
-                const float* Lt = sL[i * (i + 1) / 2 + kb];
+            static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
+                constexpr int i = decltype(I)::value;
+                const float* Lt = sL[tri(i, kb)];
 #pragma unroll
                 for (int k = 0; k < 16; ++k) t = __builtin_fmaf(-Lt[m * kTileStride + k], sV[(16 * i + k) * kMaxRhs + c], t);
-            }
+            });
             sT[m * kMaxRhs + c] = t;
-            const float* W = sL[kb * (kb + 1) / 2 + kb];
+            const float* W = sL[tri(kb, kb)];
             float x = 0.f;
 #pragma unroll
             for (int k = 0; k < 16; ++k) x = __builtin_fmaf(W[m * kTileStride + k], sT[k * kMaxRhs + c], x);
             sV[(16 * kb + m) * kMaxRhs + c] = x;
-        }
+        });
         for (int e = lane; e < NV; e += 64) sX[e] += double(sV[e]);
-        // r = b - A x in FP64; v = float(r); convergence test per column
+        // r = b - A x in FP64, A (symmetric, lower triangle stored) read in
+        // 16-column batches so the loads overlap; v = float(r)
         double rmax[kMaxRhs] = {0.0, 0.0, 0.0, 0.0}, xmax[kMaxRhs] = {0.0, 0.0, 0.0, 0.0};
         for (int h = 0; h < NT * 16; h += 64) {
             int const row = h + lane;
-            double r[kMaxRhs] = {0.0, 0.0, 0.0, 0.0};
-            if (row < n) {
+            double r[kMaxRhs];
 #pragma unroll
-                for (int cc = 0; cc < kMaxRhs; ++cc) r[cc] = cc < nrhs ? bb(row, cc) : 0.0;
-                for (int col = 0; col < n; ++col) {
-                    double const av = a_sym<PANEL>(a, row, col);
+            for (int cc = 0; cc < kMaxRhs; ++cc) r[cc] = (row < n && cc < nrhs) ? bb(row, cc) : 0.0;
+            double asum = 0.0;
+            for (int c0 = 0; c0 < NT * 16; c0 += 16) {
+                double av[16];
 #pragma unroll
-                    for (int cc = 0; cc < kMaxRhs; ++cc) r[cc] = __builtin_fma(-av, sX[col * kMaxRhs + cc], r[cc]);
+                for (int k = 0; k < 16; ++k) {
+                    int const col = c0 + k;
+                    bool const in = row < n && col < n;
+                    int const lo = row >= col ? row : col, hi = row >= col ? col : row;
+                    int64_t const off = in ? (PANEL ? int64_t(lo >> 2) * A.spacing + (lo & 3) + 4 * hi : A.off(lo, hi)) : 0;
+                    double const v = a.p[off];
+                    av[k] = in ? v : 0.0;
+                }
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    asum += __builtin_fabs(av[k]);
+#pragma unroll
+                    for (int cc = 0; cc < kMaxRhs; ++cc)
+                        r[cc] = __builtin_fma(-av[k], sX[(c0 + k) * kMaxRhs + cc], r[cc]);
                 }
             }
+            anrm = __builtin_fmax(anrm, asum);
             if (row < NT * 16)
 #pragma unroll
                 for (int cc = 0; cc < kMaxRhs; ++cc) {
@@ -203,6 +212,11 @@ __global__ __launch_bounds__(64) void sposv_ir_kernel(Mat<double> A, Mat<double>
                     xmax[cc] = __builtin_fmax(xmax[cc], __builtin_fabs(sX[row * kMaxRhs + cc]));
                 }
         }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) anrm = __builtin_fmax(anrm, __shfl_xor(anrm, o));
+        // LAPACK dsposv: ||r|| <= ||x|| ||A|| eps sqrt(n); with tol > 0:
+        // ||r|| <= tol ||x|| ||A|| (infinity norms, per right-hand side).
+        double const cte = anrm * (tol > 0.0 ? tol : 1.1102230246251565e-16 * __builtin_sqrt(double(n)));
         bool ok = true;
 #pragma unroll
         for (int cc = 0; cc < kMaxRhs; ++cc) {
@@ -247,39 +261,39 @@ int launch_fill_i32(hipStream_t s, int32_t* p, int32_t v, int64_t count) {
 
 template <int NT, bool BF16>
 static int launch_ir(hipStream_t s, Mat<double> A, Mat<double> B, Mat<double> X, int n, int nrhs, int itmax,
-                     int64_t batch, int32_t* info, int32_t* iter) {
+                     double tol, int64_t batch, int32_t* info, int32_t* iter) {
     if (A.panel && !A.trans)
         hipLaunchKernelGGL((sposv_ir_kernel<NT, true, BF16>), dim3(unsigned(batch)), dim3(64), 0, s, A, B, X, n, nrhs,
-                           itmax, info, iter);
+                           itmax, tol, info, iter);
     else
         hipLaunchKernelGGL((sposv_ir_kernel<NT, false, BF16>), dim3(unsigned(batch)), dim3(64), 0, s, A, B, X, n,
-                           nrhs, itmax, info, iter);
+                           nrhs, itmax, tol, info, iter);
     return check_launch("sposv_ir_kernel");
 }
 
 template <bool BF16>
-static int launch_ir_n(hipStream_t s, Mat<double> A, Mat<double> B, Mat<double> X, int itmax, int64_t batch,
-                       int32_t* info, int32_t* iter) {
+static int launch_ir_n(hipStream_t s, Mat<double> A, Mat<double> B, Mat<double> X, int itmax, double tol,
+                       int64_t batch, int32_t* info, int32_t* iter) {
     int const n = A.rows, nrhs = X.cols;
     switch ((n + 15) / 16) {
         case 0:
-        case 1: return launch_ir<1, BF16>(s, A, B, X, n, nrhs, itmax, batch, info, iter);
-        case 2: return launch_ir<2, BF16>(s, A, B, X, n, nrhs, itmax, batch, info, iter);
-        case 3: return launch_ir<3, BF16>(s, A, B, X, n, nrhs, itmax, batch, info, iter);
-        case 4: return launch_ir<4, BF16>(s, A, B, X, n, nrhs, itmax, batch, info, iter);
-        case 5: return launch_ir<5, BF16>(s, A, B, X, n, nrhs, itmax, batch, info, iter);
-        case 6: return launch_ir<6, BF16>(s, A, B, X, n, nrhs, itmax, batch, info, iter);
-        case 7: return launch_ir<7, BF16>(s, A, B, X, n, nrhs, itmax, batch, info, iter);
-        default: return launch_ir<8, BF16>(s, A, B, X, n, nrhs, itmax, batch, info, iter);
+        case 1: return launch_ir<1, BF16>(s, A, B, X, n, nrhs, itmax, tol, batch, info, iter);
+        case 2: return launch_ir<2, BF16>(s, A, B, X, n, nrhs, itmax, tol, batch, info, iter);
+        case 3: return launch_ir<3, BF16>(s, A, B, X, n, nrhs, itmax, tol, batch, info, iter);
+        case 4: return launch_ir<4, BF16>(s, A, B, X, n, nrhs, itmax, tol, batch, info, iter);
+        case 5: return launch_ir<5, BF16>(s, A, B, X, n, nrhs, itmax, tol, batch, info, iter);
+        case 6: return launch_ir<6, BF16>(s, A, B, X, n, nrhs, itmax, tol, batch, info, iter);
+        case 7: return launch_ir<7, BF16>(s, A, B, X, n, nrhs, itmax, tol, batch, info, iter);
+        default: return launch_ir<8, BF16>(s, A, B, X, n, nrhs, itmax, tol, batch, info, iter);
     }
 }
 
-int launch_sposv_ir(hipStream_t s, int bf16, Mat<double> A, Mat<double> B, Mat<double> X, int itmax, int64_t batch,
-                    int32_t* info, int32_t* iter) {
+int launch_sposv_ir(hipStream_t s, int bf16, Mat<double> A, Mat<double> B, Mat<double> X, int itmax, double tol,
+                    int64_t batch, int32_t* info, int32_t* iter) {
     if (A.rows != A.cols || A.rows > 128 || X.cols > kMaxRhs || A.rows == 0 || X.cols == 0) return 1;
     if (batch == 0) return 0;
-    return bf16 ? launch_ir_n<true>(s, A, B, X, itmax, batch, info, iter)
-                : launch_ir_n<false>(s, A, B, X, itmax, batch, info, iter);
+    return bf16 ? launch_ir_n<true>(s, A, B, X, itmax, tol, batch, info, iter)
+                : launch_ir_n<false>(s, A, B, X, itmax, tol, batch, info, iter);
 }
 
 }  // namespace blast_hip

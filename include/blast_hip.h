@@ -137,14 +137,15 @@ int blast_hip_dposv_batched(void* stream, const blast_matrix_t* A, const blast_m
  * BF16 products and FP32 accumulation (precision = 1); the residual
  * r = b - A x, the update x += d and the convergence test
  * max|r| <= max|x| * ||A||_inf * eps * sqrt(n) (per right-hand side) are
- * FP64.  itmax bounds the refinement steps (LAPACK uses 30).
+ * FP64; tol > 0 replaces eps*sqrt(n) in that test (tol = 0: LAPACK's test).
+ * itmax bounds the refinement steps (LAPACK uses 30).
  * iter (device int32 per instance, required): refinement steps taken, or
  * -1 (FP32 factorisation broke down), -2 (no convergence), -3 (shape not
  * covered by the mixed kernel: n > 128 or nrhs > 4).  Instances with
  * iter < 0 are solved by the FP64 path, which writes their Cholesky factor
  * into A (lower triangle) as LAPACK dsposv does; otherwise A is unchanged.
  * info as for blast_hip_dposv_batched.  X may alias B. */
-int blast_hip_dsposv_batched(void* stream, int precision, int itmax, const blast_matrix_t* A,
+int blast_hip_dsposv_batched(void* stream, int precision, int itmax, double tol, const blast_matrix_t* A,
                              const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch, int32_t* info,
                              int32_t* iter);
 
