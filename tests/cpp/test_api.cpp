@@ -1,0 +1,198 @@
+// Drives the C++ front end (include/blast/hip/Blast.hpp) on the GPU the way a
+// BLAST user would call BLAST: same function names, argument order and
+// exceptions.  The checks are plain loops in this file.  Prints "ALL OK".
+#include <blast/hip/Blast.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+static int failures = 0;
+#define CHECK(cond)                                                              \
+    do {                                                                         \
+        if (!(cond)) {                                                           \
+            std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);          \
+            ++failures;                                                          \
+        }                                                                        \
+    } while (0)
+
+template <class MT>
+void fill_dyadic(MT& m, std::mt19937& g) {
+    std::uniform_int_distribution<int> d(-256, 256);
+    for (std::size_t j = 0; j < m.columns(); ++j)
+        for (std::size_t i = 0; i < m.rows(); ++i) m(i, j) = d(g) / 256.0;
+}
+
+template <class MT>
+void make_spd(MT& a, std::mt19937& g) {
+    std::size_t const n = a.rows();
+    std::normal_distribution<double> d;
+    std::vector<double> gm(n * n);
+    for (auto& v : gm) v = d(g);
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t j = 0; j < n; ++j) {
+            double s = i == j ? double(n) : 0.0;
+            for (std::size_t k = 0; k < n; ++k) s += gm[i * n + k] * gm[j * n + k];
+            a(i, j) = s;
+        }
+}
+
+int main() {
+    std::mt19937 g(42);
+    // gemm NT on static panel matrices, exact dyadic data: bit-exact check.
+    {
+        blast::StaticPanelMatrix<double, 12, 12> A, B, C, D;
+        fill_dyadic(A, g), fill_dyadic(B, g), fill_dyadic(C, g);
+        blast::gemm(1.5, A, blast::trans(B), -0.5, C, D);
+        for (std::size_t i = 0; i < 12; ++i)
+            for (std::size_t j = 0; j < 12; ++j) {
+                double s = 0.0;
+                for (std::size_t k = 0; k < 12; ++k) s += A(i, k) * B(j, k);
+                CHECK(D(i, j) == 1.5 * s - 0.5 * C(i, j));
+            }
+        blast::StaticPanelMatrix<double, 12, 11> Bad;
+        bool thrown = false;
+        try {
+            blast::gemm(1.0, A, Bad, 1.0, C, D);
+        } catch (std::invalid_argument const&) {
+            thrown = true;
+        }
+        CHECK(thrown);
+    }
+    // potrf + the two trsm of a Cholesky solve on dynamic panel/dense matrices.
+    {
+        std::size_t const n = 20, r = 3;
+        blast::DynamicPanelMatrix<double> A(n, n), L(n, n);
+        blast::DynamicMatrix<double> B(n, r), X(n, r);
+        make_spd(A, g);
+        fill_dyadic(B, g);
+        blast::potrf(A, L);
+        double err = 0.0, na = 0.0;
+        for (std::size_t i = 0; i < n; ++i)
+            for (std::size_t j = 0; j <= i; ++j) {
+                double s = 0.0;
+                for (std::size_t k = 0; k <= j; ++k) s += L(i, k) * L(j, k);
+                err = std::fmax(err, std::fabs(s - A(i, j)));
+                na = std::fmax(na, std::fabs(A(i, j)));
+            }
+        CHECK(err <= 1e-13 * na);
+        blast::trsm<blast::UpLo::Lower, false>(L, B, X);
+        blast::trsm<blast::UpLo::Upper, false>(blast::trans(L), X, X);
+        double res = 0.0;
+        for (std::size_t j = 0; j < r; ++j)
+            for (std::size_t i = 0; i < n; ++i) {
+                double s = 0.0;
+                for (std::size_t k = 0; k < n; ++k) s += A(i, k) * X(k, j);
+                res = std::fmax(res, std::fabs(s - B(i, j)));
+            }
+        CHECK(res <= 1e-12);
+        bool thrown = false;
+        blast::DynamicPanelMatrix<double> Wide(4, 8), Wl(4, 8);
+        try {
+            blast::potrf(Wide, Wl);
+        } catch (std::invalid_argument const&) {
+            thrown = true;
+        }
+        CHECK(thrown);
+    }
+    // getrf with partial pivoting; singular matrix throws.
+    {
+        std::size_t const n = 9;
+        blast::DynamicMatrix<double> A(n, n), A0(n, n);
+        std::uniform_real_distribution<double> u(-1, 1);
+        for (std::size_t j = 0; j < n; ++j)
+            for (std::size_t i = 0; i < n; ++i) A(i, j) = A0(i, j) = u(g);
+        std::size_t ipiv[9];
+        blast::getrf(A, ipiv);
+        std::vector<double> P(n * n);
+        for (std::size_t i = 0; i < n; ++i)
+            for (std::size_t j = 0; j < n; ++j) P[i * n + j] = A0(i, j);
+        for (std::size_t k = 0; k < n; ++k)
+            if (ipiv[k] != k)
+                for (std::size_t j = 0; j < n; ++j) std::swap(P[k * n + j], P[ipiv[k] * n + j]);
+        double err = 0.0;
+        for (std::size_t i = 0; i < n; ++i)
+            for (std::size_t j = 0; j < n; ++j) {
+                double s = 0.0;
+                for (std::size_t k = 0; k <= std::min(i, j); ++k) s += (k == i ? 1.0 : A(i, k)) * A(k, j);
+                err = std::fmax(err, std::fabs(s - P[i * n + j]));
+            }
+        CHECK(err <= 1e-13);
+        blast::DynamicMatrix<double> S(4, 4);
+        for (std::size_t j = 0; j < 4; ++j)
+            for (std::size_t i = 0; i < 4; ++i) S(i, j) = 1.0;
+        bool thrown = false;
+        try {
+            blast::getrf(S, ipiv);
+        } catch (std::invalid_argument const&) {
+            thrown = true;
+        }
+        CHECK(thrown);
+    }
+    // syrkLower and the unimplemented trmm form.
+    {
+        blast::DynamicMatrix<double> A(7, 5), C(7, 7), D(7, 7);
+        fill_dyadic(A, g), fill_dyadic(C, g);
+        blast::syrkLower(0.5, A, 2.0, C, D);
+        for (std::size_t j = 0; j < 7; ++j)
+            for (std::size_t i = j; i < 7; ++i) {
+                double s = 0.0;
+                for (std::size_t k = 0; k < 5; ++k) s += A(i, k) * A(j, k);
+                CHECK(D(i, j) == 0.5 * s + 2.0 * C(i, j));
+            }
+        bool thrown = false;
+        blast::hip::DeviceArray<double> buf(64);
+        blast::StaticMatrix<double, 4, 4> proto;
+        auto b = blast::hip::Batch<double>::like(proto, buf.get(), 1);
+        try {
+            blast::trmm<double>(1.0, b, blast::UpLo::Upper, true, b, b);
+        } catch (std::logic_error const&) {
+            thrown = true;
+        }
+        CHECK(thrown);
+    }
+    // Batched Cholesky solve on device memory: 64 instances of n = 32.
+    {
+        std::size_t const n = 32, r = 2, batch = 64;
+        blast::DynamicPanelMatrix<double> protoA(n, n), protoB(n, r);
+        std::size_t const sa = blast::nextMultiple(protoA.storageSize(), 8), sb = blast::nextMultiple(protoB.storageSize(), 8);
+        std::vector<double> ha(batch * sa), hb(batch * sb), hx(batch * sb);
+        std::vector<blast::DynamicPanelMatrix<double>> As;
+        for (std::size_t k = 0; k < batch; ++k) {
+            blast::DynamicPanelMatrix<double> a(n, n), bm(n, r);
+            make_spd(a, g);
+            fill_dyadic(bm, g);
+            std::memcpy(&ha[k * sa], a.data(), a.storageSize() * 8);
+            std::memcpy(&hb[k * sb], bm.data(), bm.storageSize() * 8);
+        }
+        blast::hip::DeviceArray<double> da(ha.size()), dl(ha.size()), db(hb.size()), dx(hb.size());
+        blast::hip::DeviceArray<std::int32_t> info(batch);
+        blast::hip::check_hip(hipMemcpy(da.get(), ha.data(), ha.size() * 8, hipMemcpyHostToDevice));
+        blast::hip::check_hip(hipMemcpy(db.get(), hb.data(), hb.size() * 8, hipMemcpyHostToDevice));
+        auto A = blast::hip::Batch<double>::like(protoA, da.get(), batch);
+        auto L = blast::hip::Batch<double>::like(protoA, dl.get(), batch);
+        auto B = blast::hip::Batch<double>::like(protoB, db.get(), batch);
+        auto X = blast::hip::Batch<double>::like(protoB, dx.get(), batch);
+        blast::posv(A, L, B, X, info.get());
+        blast::hip::check_hip(hipDeviceSynchronize());
+        blast::hip::check_hip(hipMemcpy(hx.data(), dx.get(), hx.size() * 8, hipMemcpyDeviceToHost));
+        double worst = 0.0;
+        for (std::size_t k = 0; k < batch; ++k) {
+            blast::DynamicPanelMatrix<double> a(n, n), bm(n, r), xm(n, r);
+            std::memcpy(a.data(), &ha[k * sa], a.storageSize() * 8);
+            std::memcpy(bm.data(), &hb[k * sb], bm.storageSize() * 8);
+            std::memcpy(xm.data(), &hx[k * sb], xm.storageSize() * 8);
+            for (std::size_t j = 0; j < r; ++j)
+                for (std::size_t i = 0; i < n; ++i) {
+                    double s = 0.0;
+                    for (std::size_t q = 0; q < n; ++q) s += (i >= q ? a(i, q) : a(q, i)) * xm(q, j);
+                    worst = std::fmax(worst, std::fabs(s - bm(i, j)));
+                }
+        }
+        CHECK(worst <= 1e-12);
+    }
+    if (failures == 0) std::printf("ALL OK\n");
+    return failures ? 1 : 0;
+}
