@@ -131,6 +131,32 @@ def time_steps(fn, steps, warmup):
     return start.elapsed_time(stop) / 1e3 / steps  # seconds per step
 
 
+def gemm_workload(steps, warmup, batch=4096, n=64):
+    """Batched dgemm NT D = 1.5 A B^T - 0.5 C, m = n = k = 64, StaticPanelMatrix
+    layout, U(-1,1) inputs (the BLAST panel gemm benchmark shape at n = 64)."""
+    import torch
+
+    from blast_amd import capi
+    from blast_amd.layout import Layout
+
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    la = Layout.make(n, n)
+    bufs = [torch.rand(batch * la.batch_stride, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
+            for _ in range(4)]
+    A, B, C, D = (capi.Operand(la, t) for t in bufs)
+    Bt = B.t()
+
+    def step():
+        capi.gemm(1.5, A, Bt, -0.5, C, D, batch)
+
+    t = time_steps(step, steps, warmup)
+    flops = 2 * n ** 3 * batch
+    hbm = 4 * n * n * 8 * batch
+    return {"gflops": flops / t / 1e9, "instances_per_s": batch / t, "ms_per_step": t * 1e3,
+            "hbm_gbs_algorithmic": hbm / t / 1e9, "hbm_frac": hbm / t / 1e9 / PEAK_HBM_GBS,
+            "kernel": "gemm_tile_kernel<double, 0>"}
+
+
 def cpu_baseline(la, lb, abuf, bbuf, target_s):
     from oracle import oracle
 
@@ -268,6 +294,7 @@ def main():
             "fp64_frac_of_peak": FLOPS_PER_INSTANCE * batch / t_plain / 1e12 / PEAK_FP64_TFLOPS,
         },
     }
+    line["config"]["other_workloads"] = {"dgemm_64_batch4096": gemm_workload(args.steps, args.warmup)}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(la, lb, abuf, bbuf, args.cpu_seconds)
     print(json.dumps(line))

@@ -9,16 +9,19 @@ namespace blast_hip {
 
 template <typename T>
 int dispatch_gemm(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D, int64_t batch) {
+    if (algorithm() == BLAST_HIP_ALGO_AUTO) return launch_gemm_tiles<T>(s, alpha, A, B, beta, C, D, batch);
     return launch_gemm_ref<T>(s, alpha, A, B, beta, C, D, batch);
 }
 
 template <typename T>
 int dispatch_syrk(hipStream_t s, T alpha, Mat<T> A, T beta, Mat<T> C, Mat<T> D, int64_t batch) {
+    if (algorithm() == BLAST_HIP_ALGO_AUTO) return launch_syrk_tiles<T>(s, alpha, A, beta, C, D, batch);
     return launch_syrk_ref<T>(s, alpha, A, beta, C, D, batch);
 }
 
 template <typename T>
 int dispatch_trmm(hipStream_t s, int side_right, T alpha, Mat<T> A, Mat<T> B, Mat<T> C, int64_t batch) {
+    if (algorithm() == BLAST_HIP_ALGO_AUTO) return launch_trmm_tiles<T>(s, side_right, alpha, A, B, C, batch);
     return launch_trmm_ref<T>(s, side_right, alpha, A, B, C, batch);
 }
 

@@ -16,6 +16,11 @@ template <typename T> int launch_trsm_ref(hipStream_t, int, int, int, T, Mat<T>,
 template <typename T> int launch_potrf_ref(hipStream_t, Mat<T>, Mat<T>, int64_t, int32_t*);
 template <typename T> int launch_getrf_ref(hipStream_t, Mat<T>, int64_t*, int64_t, int64_t, int32_t*);
 
+// gemm_tiles.hip (MFMA tiles, any size and layout)
+template <typename T> int launch_gemm_tiles(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t);
+template <typename T> int launch_syrk_tiles(hipStream_t, T, Mat<T>, T, Mat<T>, Mat<T>, int64_t);
+template <typename T> int launch_trmm_tiles(hipStream_t, int, T, Mat<T>, Mat<T>, Mat<T>, int64_t);
+
 // posv_tiles.hip: returns 1 when the shape is not covered.  only_if: when
 // non-null, only instances b with only_if[b] < 0 are processed.
 int launch_posv_tiles(hipStream_t, Mat<double>, Mat<double>, Mat<double>, Mat<double>, int64_t, int32_t*,

@@ -1,0 +1,103 @@
+// Batched D = alpha*op(A)*op(B) + beta*C on MFMA tiles (FP64: v_mfma_f64_16x16x4f64,
+// FP32: v_mfma_f32_16x16x4f32), one 256-thread workgroup per instance.
+//
+// Each wavefront owns 16-row strips of D (strip R = wavefront, +4, ...) and
+// computes up to four 16x16 tiles of a strip at once: per K-chunk of 4 it
+// loads one A-operand element per lane (reused across the four tiles) and one
+// B-operand element per tile.  The epilogue is BLAST's: d = fma(beta, c,
+// alpha*acc) (register_matrix/Gemm.hpp:89-91).  The same kernel serves
+//   syrkLower  D = alpha*A*A^T + beta*C, lower triangle stored
+//              (dense/Syrk.hpp:15-145; B is the transposed view of A);
+//   trmm       C = alpha*A*B with A upper (left form) or alpha*B*A with A
+//              lower (right form) (algorithm/Trmm.hpp:93-242): the
+//              triangular operand is masked on load, beta = 0.
+// Summation order differs from BLAST's per-element FMA chain (agreement to
+// backward error; bit-exact on exactly representable data).
+
+#include "dispatch.hpp"
+#include "launch.hpp"
+#include "tiles.hpp"
+
+namespace blast_hip {
+
+enum GemmMode : int { kGemm = 0, kSyrkLower = 1, kTrmmLeftUpper = 2, kTrmmRightLower = 3 };
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void gemm_tile_kernel(T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D,
+                                                        int K) {
+    using M = Mfma<T>;
+    using acc_t = typename M::acc_t;
+    int64_t const b = blockIdx.x;
+    int const lane = threadIdx.x & 63;
+    int const w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    Ref<T> a(A, b), bb(B, b), c(C, b), d(D, b);
+    int const Mr = D.rows, Nc = D.cols;
+    int const MT = (Mr + 15) / 16, NTt = (Nc + 15) / 16, KQ = (K + 3) / 4;
+    int const m = lane & 15, g = lane >> 4;
+
+    for (int R = w; R < MT; R += 4) {
+        int const row = 16 * R + m;  // A-operand row of this lane
+        for (int T0 = 0; T0 < NTt; T0 += 4) {
+            if (MODE == kSyrkLower && 16 * T0 > 16 * R + 15) break;  // strictly upper tiles
+            acc_t acc[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = acc_t{};
+            for (int q = 0; q < KQ; ++q) {
+                int const k = 4 * q + g;
+                bool const ina = row < Mr && k < K && (MODE != kTrmmLeftUpper || k >= row);
+                T const av = ina ? a(row, k) : T(0);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    int const col = 16 * (T0 + t) + m;  // B-operand column of this lane
+                    bool const inb = col < Nc && k < K && (MODE != kTrmmRightLower || k >= col);
+                    T const bv = inb ? bb(k, col) : T(0);
+                    acc[t] = M::mma(av, bv, acc[t]);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    int const i = 16 * R + M::row(lane, r), j = 16 * (T0 + t) + acc_col(lane);
+                    if (i < Mr && j < Nc && (MODE != kSyrkLower || i >= j)) {
+                        T const ab = acc[t][r] * alpha;
+                        d(i, j) = (MODE == kTrmmLeftUpper || MODE == kTrmmRightLower) ? ab
+                                                                                       : __builtin_fma(beta, c(i, j), ab);
+                    }
+                }
+        }
+    }
+}
+
+template <typename T, int MODE>
+static int launch_mode(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D, int K, int64_t batch) {
+    if (batch == 0 || D.rows == 0 || D.cols == 0) return 0;
+    hipLaunchKernelGGL((gemm_tile_kernel<T, MODE>), dim3(unsigned(batch)), dim3(256), 0, s, alpha, A, B, beta, C, D, K);
+    return check_launch("gemm_tile_kernel");
+}
+
+template <typename T>
+int launch_gemm_tiles(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D, int64_t batch) {
+    return launch_mode<T, kGemm>(s, alpha, A, B, beta, C, D, A.cols, batch);
+}
+
+template <typename T>
+int launch_syrk_tiles(hipStream_t s, T alpha, Mat<T> A, T beta, Mat<T> C, Mat<T> D, int64_t batch) {
+    return launch_mode<T, kSyrkLower>(s, alpha, A, transposed(A), beta, C, D, A.cols, batch);
+}
+
+// side_right = 0: C = alpha*A*B (A upper); 1: C = alpha*B*A (A lower).
+template <typename T>
+int launch_trmm_tiles(hipStream_t s, int side_right, T alpha, Mat<T> A, Mat<T> B, Mat<T> C, int64_t batch) {
+    if (!side_right) return launch_mode<T, kTrmmLeftUpper>(s, alpha, A, B, T(0), C, C, A.cols, batch);
+    return launch_mode<T, kTrmmRightLower>(s, alpha, B, A, T(0), C, C, B.cols, batch);
+}
+
+#define INSTANTIATE(T)                                                                              \
+    template int launch_gemm_tiles<T>(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t); \
+    template int launch_syrk_tiles<T>(hipStream_t, T, Mat<T>, T, Mat<T>, Mat<T>, int64_t);         \
+    template int launch_trmm_tiles<T>(hipStream_t, int, T, Mat<T>, Mat<T>, Mat<T>, int64_t);
+INSTANTIATE(double)
+INSTANTIATE(float)
+
+}  // namespace blast_hip

@@ -148,3 +148,64 @@ def test_sposv_uncovered_shape_uses_fp64():
     capi.sposv(0, A, B, X, batch, it)
     assert np.all(it.cpu().numpy() == -3)
     assert np.all(solve_backward_error(a, X.logical(batch), b) < 1e-15)
+
+
+# ------------------------------------------------------------------ gemm / syrk / trmm on MFMA tiles
+def _lay(rows, cols, dtype, panel, trans=False):
+    return Layout.make(cols, rows, dtype, panel=panel).t() if trans else Layout.make(rows, cols, dtype, panel=panel)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-14), (np.float32, 2e-6)])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (5, 7, 3), (16, 16, 16), (64, 64, 64), (33, 70, 9), (100, 20, 129)])
+@pytest.mark.parametrize("ta,tb,panel", [(0, 1, 1), (0, 0, 0), (1, 1, 1), (1, 0, 0)])
+def test_gemm_tiles(dtype, tol, m, n, k, ta, tb, panel):
+    rng = np.random.default_rng(m + 31 * n + 7 * k)
+    batch = 5
+    a, b, c = (rng.uniform(-1, 1, s) for s in ((batch, m, k), (batch, k, n), (batch, m, n)))
+    la, lb, lc = _lay(m, k, dtype, panel, ta), _lay(k, n, dtype, not panel, tb), _lay(m, n, dtype, panel)
+    D = capi.Operand.empty(lc, batch)
+    capi.gemm(1.5, dev(la, la.pack(a)), dev(lb, lb.pack(b)), -0.5, dev(lc, lc.pack(c)), D, batch)
+    ref = 1.5 * (a @ b) - 0.5 * c
+    err = np.abs(D.logical(batch) - ref).max() / (k * max(np.abs(a).max() * np.abs(b).max(), 1e-30) + 1)
+    assert err < tol
+
+
+def test_gemm_tiles_exact_dyadic_and_inplace():
+    from tests.numerics import dyadic
+
+    rng = np.random.default_rng(1)
+    batch, m = 6, 64
+    a, b, c = dyadic(rng, (batch, m, m)), dyadic(rng, (batch, m, m)), dyadic(rng, (batch, m, m))
+    la = Layout.make(m, m)
+    C = dev(la, la.pack(c))
+    capi.gemm(1.5, dev(la, la.pack(a)), dev(la.t(), la.t().pack(b)), -0.5, C, C, batch)  # NT, C aliases D
+    np.testing.assert_array_equal(C.logical(batch), 1.5 * (a @ b) - 0.5 * c)
+
+
+@pytest.mark.parametrize("m,k", [(1, 1), (9, 4), (37, 11), (64, 64)])
+def test_syrk_tiles(m, k):
+    rng = np.random.default_rng(m * k)
+    batch = 4
+    a, c = rng.uniform(-1, 1, (batch, m, k)), rng.uniform(-1, 1, (batch, m, m))
+    la, lc = Layout.make(m, k, panel=False), Layout.make(m, m, panel=False)
+    D = capi.Operand.empty(lc, batch, fill=7.0)
+    capi.syrk_lower(0.5, dev(la, la.pack(a)), 2.0, dev(lc, lc.pack(c)), D, batch)
+    got = D.logical(batch)
+    ref = 0.5 * a @ a.transpose(0, 2, 1) + 2.0 * c
+    low = np.tril(np.ones((m, m), bool))
+    np.testing.assert_allclose(got[:, low], ref[:, low], rtol=0, atol=1e-13 * max(k, 1))
+    assert np.all(got[:, ~low] == 7.0)
+
+
+@pytest.mark.parametrize("side_right", [0, 1])
+@pytest.mark.parametrize("m,n", [(1, 1), (7, 5), (20, 33)])
+def test_trmm_tiles(side_right, m, n):
+    rng = np.random.default_rng(m * n + 3 * side_right)
+    batch = 3
+    na = n if side_right else m
+    a, b = rng.uniform(-1, 1, (batch, na, na)), rng.uniform(-1, 1, (batch, m, n))
+    la, lb = Layout.make(na, na), Layout.make(m, n)
+    C = capi.Operand.empty(lb, batch)
+    capi.trmm(side_right, not side_right, 0.75, dev(la, la.pack(a)), dev(lb, lb.pack(b)), C, batch)
+    ref = 0.75 * (b @ np.tril(a) if side_right else np.triu(a) @ b)
+    np.testing.assert_allclose(C.logical(batch), ref, rtol=0, atol=1e-13 * na)
