@@ -65,6 +65,13 @@ def lib():
             getattr(L, f"blast_hip_{p}getrf_batched").argtypes = [vp, P, vp, i64, i64, i32p]
         L.blast_hip_dposv_batched.argtypes = [vp, P, P, P, P, i64, i32p]
         L.blast_hip_dsposv_batched.argtypes = [vp, c_int, c_int, ctypes.c_double, P, P, P, i64, i32p, i32p]
+        for p, real in (("d", ctypes.c_double), ("s", ctypes.c_float)):
+            getattr(L, f"blast_hip_{p}gemm_backward_batched").argtypes = [vp, real, P, P, real, P, P, P, P, i64]
+            getattr(L, f"blast_hip_{p}trsm_backward_batched").argtypes = [vp, c_int, P, P, P, P, P, i64]
+            getattr(L, f"blast_hip_{p}potrf_backward_batched").argtypes = [vp, P, P, P, i64]
+            getattr(L, f"blast_hip_{p}posv_backward_batched").argtypes = [vp, c_int, P, P, P, P, P, i64, i32p]
+            getattr(L, f"blast_hip_{p}getrs_batched").argtypes = [vp, P, vp, i64, P, P, i64]
+            getattr(L, f"blast_hip_{p}getrs_backward_batched").argtypes = [vp, P, vp, i64, P, P, P, P, i64]
         L.blast_hip_set_algorithm.argtypes = [c_int]
         L.blast_hip_last_error.restype = ctypes.c_char_p
         L.blast_hip_version.restype = ctypes.c_char_p
@@ -191,3 +198,42 @@ def sposv(precision, A, B, X, batch, iter_, info=None, itmax=30, tol=0.0, stream
     name = "blast_hip_dsposv_batched"
     return check(name, lib().blast_hip_dsposv_batched(_stream(stream), int(precision), int(itmax), float(tol), _ref(A),
                                                       _ref(B), _ref(X), batch, _info_ptr(info), _info_ptr(iter_)))
+
+
+def _oref(op):
+    return None if op is None else _ref(op)
+
+
+def gemm_backward(alpha, A, B, beta, Db, Ab, Bb, Cb, batch, stream=None):
+    name = f"blast_hip_{_prefix(Db.layout.dtype)}gemm_backward_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), alpha, _ref(A), _ref(B), beta, _ref(Db), _oref(Ab),
+                                            _oref(Bb), _oref(Cb), batch))
+
+
+def trsm_backward(upper, A, X, Xb, Ab, Bb, batch, stream=None):
+    name = f"blast_hip_{_prefix(X.layout.dtype)}trsm_backward_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), int(upper), _ref(A), _ref(X), _ref(Xb), _oref(Ab),
+                                            _ref(Bb), batch))
+
+
+def potrf_backward(L, Lb, Ab, batch, stream=None):
+    name = f"blast_hip_{_prefix(L.layout.dtype)}potrf_backward_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), _ref(L), _ref(Lb), _ref(Ab), batch))
+
+
+def posv_backward(recompute, F, X, Xb, Ab, Bb, batch, info=None, stream=None):
+    name = f"blast_hip_{_prefix(X.layout.dtype)}posv_backward_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), int(recompute), _ref(F), _ref(X), _ref(Xb), _ref(Ab),
+                                            _ref(Bb), batch, _info_ptr(info)))
+
+
+def getrs(LU, ipiv, ipiv_stride, B, X, batch, stream=None):
+    name = f"blast_hip_{_prefix(X.layout.dtype)}getrs_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), _ref(LU), ipiv.data_ptr(), ipiv_stride, _ref(B),
+                                            _ref(X), batch))
+
+
+def getrs_backward(LU, ipiv, ipiv_stride, X, Xb, Ab, Bb, batch, stream=None):
+    name = f"blast_hip_{_prefix(X.layout.dtype)}getrs_backward_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), _ref(LU), ipiv.data_ptr(), ipiv_stride, _ref(X),
+                                            _ref(Xb), _oref(Ab), _ref(Bb), batch))

@@ -171,6 +171,127 @@ int posv_entry(void* stream, const blast_matrix_t* A, const blast_matrix_t* L, c
                             Mat<T>::from(*X), batch, info);
 }
 
+template <typename T>
+Mat<T> opt(const blast_matrix_t* m) {
+    if (m) return Mat<T>::from(*m);
+    Mat<T> r{};
+    return r;
+}
+
+#define CHECK_SHAPE(cond, code, msg) \
+    if (!(cond)) {                   \
+        set_error("%s", msg);        \
+        return code;                 \
+    }
+
+template <typename T>
+int gemm_backward_entry(void* s, T alpha, const blast_matrix_t* A, const blast_matrix_t* B, T beta,
+                        const blast_matrix_t* Db, const blast_matrix_t* Ab, const blast_matrix_t* Bb,
+                        const blast_matrix_t* Cb, int64_t batch) {
+    if (!valid<T>(A, "A")) return -3;
+    if (!valid<T>(B, "B")) return -4;
+    if (!valid<T>(Db, "Dbar")) return -6;
+    if (Ab && !valid<T>(Ab, "Abar")) return -7;
+    if (Bb && !valid<T>(Bb, "Bbar")) return -8;
+    if (Cb && !valid<T>(Cb, "Cbar")) return -9;
+    if (!valid_batch(batch)) return -10;
+    CHECK_SHAPE(B->rows == A->cols, -4, "gemm_backward: Matrix sizes do not match (B)");
+    CHECK_SHAPE(Db->rows == A->rows && Db->cols == B->cols, -6, "gemm_backward: Matrix sizes do not match (Dbar)");
+    CHECK_SHAPE(!Ab || (Ab->rows == A->rows && Ab->cols == A->cols), -7, "gemm_backward: Abar shape");
+    CHECK_SHAPE(!Bb || (Bb->rows == B->rows && Bb->cols == B->cols), -8, "gemm_backward: Bbar shape");
+    CHECK_SHAPE(!Cb || (Cb->rows == Db->rows && Cb->cols == Db->cols), -9, "gemm_backward: Cbar shape");
+    (void)hipGetLastError();
+    return gemm_backward<T>(hipStream_t(s), alpha, Mat<T>::from(*A), Mat<T>::from(*B), beta, Mat<T>::from(*Db),
+                            opt<T>(Ab), opt<T>(Bb), opt<T>(Cb), batch);
+}
+
+template <typename T>
+int trsm_backward_entry(void* s, int upper, const blast_matrix_t* A, const blast_matrix_t* X,
+                        const blast_matrix_t* Xb, const blast_matrix_t* Ab, const blast_matrix_t* Bb, int64_t batch) {
+    if (!valid<T>(A, "A")) return -3;
+    if (!valid<T>(X, "X")) return -4;
+    if (!valid<T>(Xb, "Xbar")) return -5;
+    if (Ab && !valid<T>(Ab, "Abar")) return -6;
+    if (!valid<T>(Bb, "Bbar")) return -7;
+    if (!valid_batch(batch)) return -8;
+    CHECK_SHAPE(A->rows == A->cols && A->rows == X->rows, -3, "trsm_backward: Invalid argument size (A)");
+    CHECK_SHAPE(Xb->rows == X->rows && Xb->cols == X->cols, -5, "trsm_backward: Xbar shape");
+    CHECK_SHAPE(!Ab || (Ab->rows == A->rows && Ab->cols == A->cols), -6, "trsm_backward: Abar shape");
+    CHECK_SHAPE(Bb->rows == X->rows && Bb->cols == X->cols, -7, "trsm_backward: Bbar shape");
+    (void)hipGetLastError();
+    return trsm_backward<T>(hipStream_t(s), upper, Mat<T>::from(*A), Mat<T>::from(*X), Mat<T>::from(*Xb), opt<T>(Ab),
+                            Mat<T>::from(*Bb), batch);
+}
+
+template <typename T>
+int potrf_backward_entry(void* s, const blast_matrix_t* L, const blast_matrix_t* Lb, const blast_matrix_t* Ab,
+                         int64_t batch) {
+    if (!valid<T>(L, "L")) return -2;
+    if (!valid<T>(Lb, "Lbar")) return -3;
+    if (!valid<T>(Ab, "Abar")) return -4;
+    if (!valid_batch(batch)) return -5;
+    CHECK_SHAPE(L->rows == L->cols, -2, "potrf_backward: L must be square");
+    CHECK_SHAPE(Lb->rows == L->rows && Lb->cols == L->cols, -3, "potrf_backward: Lbar shape");
+    CHECK_SHAPE(Ab->rows == L->rows && Ab->cols == L->cols, -4, "potrf_backward: Abar shape");
+    (void)hipGetLastError();
+    return potrf_backward<T>(hipStream_t(s), Mat<T>::from(*L), Mat<T>::from(*Lb), Mat<T>::from(*Ab), batch);
+}
+
+template <typename T>
+int posv_backward_entry(void* s, int recompute, const blast_matrix_t* F, const blast_matrix_t* X,
+                        const blast_matrix_t* Xb, const blast_matrix_t* Ab, const blast_matrix_t* Bb, int64_t batch,
+                        int32_t* info) {
+    if (!valid<T>(F, "F")) return -3;
+    if (!valid<T>(X, "X")) return -4;
+    if (!valid<T>(Xb, "Xbar")) return -5;
+    if (!valid<T>(Ab, "Abar")) return -6;
+    if (!valid<T>(Bb, "Bbar")) return -7;
+    if (!valid_batch(batch)) return -8;
+    CHECK_SHAPE(F->rows == F->cols && F->rows == X->rows, -3, "posv_backward: Invalid argument size (F)");
+    CHECK_SHAPE(Xb->rows == X->rows && Xb->cols == X->cols, -5, "posv_backward: Xbar shape");
+    CHECK_SHAPE(Ab->rows == F->rows && Ab->cols == F->cols, -6, "posv_backward: Abar shape");
+    CHECK_SHAPE(Bb->rows == X->rows && Bb->cols == X->cols, -7, "posv_backward: Bbar shape");
+    (void)hipGetLastError();
+    return posv_backward<T>(hipStream_t(s), recompute, Mat<T>::from(*F), Mat<T>::from(*X), Mat<T>::from(*Xb),
+                            Mat<T>::from(*Ab), Mat<T>::from(*Bb), batch, info);
+}
+
+template <typename T>
+int getrs_entry(void* s, const blast_matrix_t* LU, const int64_t* ipiv, int64_t ipiv_stride, const blast_matrix_t* B,
+                const blast_matrix_t* X, int64_t batch) {
+    if (!valid<T>(LU, "LU")) return -2;
+    if (LU->rows > 0 && !ipiv) { set_error("getrs: null ipiv"); return -3; }
+    if (ipiv_stride < LU->rows) { set_error("getrs: ipiv_stride < n"); return -4; }
+    if (!valid<T>(B, "B")) return -5;
+    if (!valid<T>(X, "X")) return -6;
+    if (!valid_batch(batch)) return -7;
+    CHECK_SHAPE(LU->rows == LU->cols && B->rows == LU->rows, -5, "getrs: Invalid argument size (B)");
+    CHECK_SHAPE(X->rows == B->rows && X->cols == B->cols, -6, "getrs: Invalid argument size (X)");
+    (void)hipGetLastError();
+    return getrs<T>(hipStream_t(s), Mat<T>::from(*LU), ipiv, ipiv_stride, Mat<T>::from(*B), Mat<T>::from(*X), batch);
+}
+
+template <typename T>
+int getrs_backward_entry(void* s, const blast_matrix_t* LU, const int64_t* ipiv, int64_t ipiv_stride,
+                         const blast_matrix_t* X, const blast_matrix_t* Xb, const blast_matrix_t* Ab,
+                         const blast_matrix_t* Bb, int64_t batch) {
+    if (!valid<T>(LU, "LU")) return -2;
+    if (LU->rows > 0 && !ipiv) { set_error("getrs_backward: null ipiv"); return -3; }
+    if (ipiv_stride < LU->rows) { set_error("getrs_backward: ipiv_stride < n"); return -4; }
+    if (!valid<T>(X, "X")) return -5;
+    if (!valid<T>(Xb, "Xbar")) return -6;
+    if (Ab && !valid<T>(Ab, "Abar")) return -7;
+    if (!valid<T>(Bb, "Bbar")) return -8;
+    if (!valid_batch(batch)) return -9;
+    CHECK_SHAPE(LU->rows == LU->cols && X->rows == LU->rows, -5, "getrs_backward: Invalid argument size (X)");
+    CHECK_SHAPE(Xb->rows == X->rows && Xb->cols == X->cols, -6, "getrs_backward: Xbar shape");
+    CHECK_SHAPE(!Ab || (Ab->rows == LU->rows && Ab->cols == LU->cols), -7, "getrs_backward: Abar shape");
+    CHECK_SHAPE(Bb->rows == X->rows && Bb->cols == X->cols, -8, "getrs_backward: Bbar shape");
+    (void)hipGetLastError();
+    return getrs_backward<T>(hipStream_t(s), Mat<T>::from(*LU), ipiv, ipiv_stride, Mat<T>::from(*X),
+                             Mat<T>::from(*Xb), opt<T>(Ab), Mat<T>::from(*Bb), batch);
+}
+
 int sposv_entry(void* stream, int precision, int itmax, double tol, const blast_matrix_t* A,
                 const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch, int32_t* info, int32_t* iter) {
     if (precision != 0 && precision != 1) { set_error("dsposv: precision must be 0 (FP32) or 1 (BF16)"); return -2; }
@@ -263,5 +384,39 @@ int blast_hip_dsposv_batched(void* s, int precision, int itmax, double tol, cons
                              int32_t* iter) {
     return sposv_entry(s, precision, itmax, tol, A, B, X, batch, info, iter);
 }
+
+#define BOTH(P, T)                                                                                                \
+    int blast_hip_##P##gemm_backward_batched(void* s, T alpha, const blast_matrix_t* A, const blast_matrix_t* B,    \
+                                             T beta, const blast_matrix_t* Db, const blast_matrix_t* Ab,            \
+                                             const blast_matrix_t* Bb, const blast_matrix_t* Cb, int64_t batch) {   \
+        return gemm_backward_entry<T>(s, alpha, A, B, beta, Db, Ab, Bb, Cb, batch);                                 \
+    }                                                                                                               \
+    int blast_hip_##P##trsm_backward_batched(void* s, int upper, const blast_matrix_t* A, const blast_matrix_t* X,  \
+                                             const blast_matrix_t* Xb, const blast_matrix_t* Ab,                    \
+                                             const blast_matrix_t* Bb, int64_t batch) {                             \
+        return trsm_backward_entry<T>(s, upper, A, X, Xb, Ab, Bb, batch);                                           \
+    }                                                                                                               \
+    int blast_hip_##P##potrf_backward_batched(void* s, const blast_matrix_t* L, const blast_matrix_t* Lb,           \
+                                              const blast_matrix_t* Ab, int64_t batch) {                            \
+        return potrf_backward_entry<T>(s, L, Lb, Ab, batch);                                                        \
+    }                                                                                                               \
+    int blast_hip_##P##posv_backward_batched(void* s, int recompute, const blast_matrix_t* F,                       \
+                                             const blast_matrix_t* X, const blast_matrix_t* Xb,                     \
+                                             const blast_matrix_t* Ab, const blast_matrix_t* Bb, int64_t batch,     \
+                                             int32_t* info) {                                                       \
+        return posv_backward_entry<T>(s, recompute, F, X, Xb, Ab, Bb, batch, info);                                 \
+    }                                                                                                               \
+    int blast_hip_##P##getrs_batched(void* s, const blast_matrix_t* LU, const int64_t* ipiv, int64_t ipiv_stride,   \
+                                     const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch) {             \
+        return getrs_entry<T>(s, LU, ipiv, ipiv_stride, B, X, batch);                                               \
+    }                                                                                                               \
+    int blast_hip_##P##getrs_backward_batched(void* s, const blast_matrix_t* LU, const int64_t* ipiv,               \
+                                              int64_t ipiv_stride, const blast_matrix_t* X,                         \
+                                              const blast_matrix_t* Xb, const blast_matrix_t* Ab,                   \
+                                              const blast_matrix_t* Bb, int64_t batch) {                            \
+        return getrs_backward_entry<T>(s, LU, ipiv, ipiv_stride, X, Xb, Ab, Bb, batch);                             \
+    }
+BOTH(d, double)
+BOTH(s, float)
 
 }  // extern "C"

@@ -20,6 +20,24 @@ template <typename T> int launch_getrf_ref(hipStream_t, Mat<T>, int64_t*, int64_
 template <typename T> int launch_gemm_tiles(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t);
 template <typename T> int launch_syrk_tiles(hipStream_t, T, Mat<T>, T, Mat<T>, Mat<T>, int64_t);
 template <typename T> int launch_trmm_tiles(hipStream_t, int, T, Mat<T>, Mat<T>, Mat<T>, int64_t);
+template <typename T> int launch_upper_times_lower(hipStream_t, T, Mat<T>, Mat<T>, Mat<T>, int64_t);
+template <typename T> int launch_gemm_overwrite(hipStream_t, T, Mat<T>, Mat<T>, Mat<T>, int64_t);
+
+// adjoint.hip
+template <typename T>
+int gemm_backward(hipStream_t, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> Db, Mat<T> Ab, Mat<T> Bb, Mat<T> Cb,
+                  int64_t batch);
+template <typename T>
+int trsm_backward(hipStream_t, int upper, Mat<T> A, Mat<T> X, Mat<T> Xb, Mat<T> Ab, Mat<T> Bb, int64_t batch);
+template <typename T> int potrf_backward(hipStream_t, Mat<T> L, Mat<T> Lb, Mat<T> Ab, int64_t batch);
+template <typename T>
+int posv_backward(hipStream_t, int recompute, Mat<T> F, Mat<T> X, Mat<T> Xb, Mat<T> Ab, Mat<T> Bb, int64_t batch,
+                  int32_t* info);
+template <typename T>
+int getrs(hipStream_t, Mat<T> LU, const int64_t* ipiv, int64_t ipiv_stride, Mat<T> B, Mat<T> X, int64_t batch);
+template <typename T>
+int getrs_backward(hipStream_t, Mat<T> LU, const int64_t* ipiv, int64_t ipiv_stride, Mat<T> X, Mat<T> Xb, Mat<T> Ab,
+                   Mat<T> Bb, int64_t batch);
 
 // posv_tiles.hip: returns 1 when the shape is not covered.  only_if: when
 // non-null, only instances b with only_if[b] < 0 are processed.

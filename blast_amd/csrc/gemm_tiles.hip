@@ -20,7 +20,8 @@
 
 namespace blast_hip {
 
-enum GemmMode : int { kGemm = 0, kSyrkLower = 1, kTrmmLeftUpper = 2, kTrmmRightLower = 3 };
+enum GemmMode : int { kGemm = 0, kSyrkLower = 1, kTrmmLeftUpper = 2, kTrmmRightLower = 3, kUpperTimesLower = 4,
+                     kGemmOverwrite = 5 };
 
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void gemm_tile_kernel(T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D,
@@ -44,12 +45,12 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(T alpha, Mat<T> A, Mat<T
             for (int t = 0; t < 4; ++t) acc[t] = acc_t{};
             for (int q = 0; q < KQ; ++q) {
                 int const k = 4 * q + g;
-                bool const ina = row < Mr && k < K && (MODE != kTrmmLeftUpper || k >= row);
+                bool const ina = row < Mr && k < K && ((MODE != kTrmmLeftUpper && MODE != kUpperTimesLower) || k >= row);
                 T const av = ina ? a(row, k) : T(0);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     int const col = 16 * (T0 + t) + m;  // B-operand column of this lane
-                    bool const inb = col < Nc && k < K && (MODE != kTrmmRightLower || k >= col);
+                    bool const inb = col < Nc && k < K && ((MODE != kTrmmRightLower && MODE != kUpperTimesLower) || k >= col);
                     T const bv = inb ? bb(k, col) : T(0);
                     acc[t] = M::mma(av, bv, acc[t]);
                 }
@@ -61,7 +62,9 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(T alpha, Mat<T> A, Mat<T
                     int const i = 16 * R + M::row(lane, r), j = 16 * (T0 + t) + acc_col(lane);
                     if (i < Mr && j < Nc && (MODE != kSyrkLower || i >= j)) {
                         T const ab = acc[t][r] * alpha;
-                        d(i, j) = (MODE == kTrmmLeftUpper || MODE == kTrmmRightLower) ? ab
+                        d(i, j) = (MODE == kTrmmLeftUpper || MODE == kTrmmRightLower || MODE == kUpperTimesLower ||
+                                   MODE == kGemmOverwrite)
+                                      ? ab
                                                                                        : __builtin_fma(beta, c(i, j), ab);
                     }
                 }
@@ -93,10 +96,24 @@ int launch_trmm_tiles(hipStream_t s, int side_right, T alpha, Mat<T> A, Mat<T> B
     return launch_mode<T, kTrmmRightLower>(s, alpha, B, A, T(0), C, C, B.cols, batch);
 }
 
+// D = alpha * A * B, D not read (used by the adjoints on uninitialised outputs).
+template <typename T>
+int launch_gemm_overwrite(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, Mat<T> D, int64_t batch) {
+    return launch_mode<T, kGemmOverwrite>(s, alpha, A, B, T(0), D, D, A.cols, batch);
+}
+
+// C = alpha * triu(A) * tril(B) (used by the potrf adjoint: L^T Lbar).
+template <typename T>
+int launch_upper_times_lower(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, Mat<T> C, int64_t batch) {
+    return launch_mode<T, kUpperTimesLower>(s, alpha, A, B, T(0), C, C, A.cols, batch);
+}
+
 #define INSTANTIATE(T)                                                                              \
     template int launch_gemm_tiles<T>(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t); \
     template int launch_syrk_tiles<T>(hipStream_t, T, Mat<T>, T, Mat<T>, Mat<T>, int64_t);         \
-    template int launch_trmm_tiles<T>(hipStream_t, int, T, Mat<T>, Mat<T>, Mat<T>, int64_t);
+    template int launch_trmm_tiles<T>(hipStream_t, int, T, Mat<T>, Mat<T>, Mat<T>, int64_t);              \
+    template int launch_upper_times_lower<T>(hipStream_t, T, Mat<T>, Mat<T>, Mat<T>, int64_t);           \
+    template int launch_gemm_overwrite<T>(hipStream_t, T, Mat<T>, Mat<T>, Mat<T>, int64_t);
 INSTANTIATE(double)
 INSTANTIATE(float)
 

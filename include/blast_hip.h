@@ -149,6 +149,67 @@ int blast_hip_dsposv_batched(void* stream, int precision, int itmax, double tol,
                              const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch, int32_t* info,
                              int32_t* iter);
 
+/* ---------------------------------------------------------------- adjoints
+ * Reverse-mode derivatives, forward/backward pairs of the entry points above.
+ * Not in BLAST; closed forms from Giles 2008 and Murray 2016, checked
+ * against central finite differences and an FP64 CPU reference.  Symmetric
+ * inputs (potrf, posv) are read from their lower triangle, so their adjoint
+ * is with respect to the stored lower entries (Abar_ij = G_ij + G_ji below
+ * the diagonal for a full gradient G; strict upper of Abar written as 0).
+ * Output buffers may be uninitialised; they also serve as workspace.  */
+
+/* D = alpha A B + beta C:  Abar = alpha Dbar B^T, Bbar = alpha A^T Dbar,
+ * Cbar = beta Dbar.  Abar, Bbar, Cbar may be NULL (not computed). */
+int blast_hip_dgemm_backward_batched(void* stream, double alpha, const blast_matrix_t* A, const blast_matrix_t* B,
+                                     double beta, const blast_matrix_t* Dbar, const blast_matrix_t* Abar,
+                                     const blast_matrix_t* Bbar, const blast_matrix_t* Cbar, int64_t batch);
+int blast_hip_sgemm_backward_batched(void* stream, float alpha, const blast_matrix_t* A, const blast_matrix_t* B,
+                                     float beta, const blast_matrix_t* Dbar, const blast_matrix_t* Abar,
+                                     const blast_matrix_t* Bbar, const blast_matrix_t* Cbar, int64_t batch);
+
+/* A X = B (left, non-unit, A lower or upper):  Bbar = A^{-T} Xbar,
+ * Abar = -Bbar X^T on A's triangle (other triangle 0; Abar may be NULL). */
+int blast_hip_dtrsm_backward_batched(void* stream, int upper, const blast_matrix_t* A, const blast_matrix_t* X,
+                                     const blast_matrix_t* Xbar, const blast_matrix_t* Abar,
+                                     const blast_matrix_t* Bbar, int64_t batch);
+int blast_hip_strsm_backward_batched(void* stream, int upper, const blast_matrix_t* A, const blast_matrix_t* X,
+                                     const blast_matrix_t* Xbar, const blast_matrix_t* Abar,
+                                     const blast_matrix_t* Bbar, int64_t batch);
+
+/* L = chol(A):  Abar = fold(L^{-T} Phi(L^T tril(Lbar)) L^{-1}),
+ * Phi = lower triangle with halved diagonal. */
+int blast_hip_dpotrf_backward_batched(void* stream, const blast_matrix_t* L, const blast_matrix_t* Lbar,
+                                      const blast_matrix_t* Abar, int64_t batch);
+int blast_hip_spotrf_backward_batched(void* stream, const blast_matrix_t* L, const blast_matrix_t* Lbar,
+                                      const blast_matrix_t* Abar, int64_t batch);
+
+/* X = A^{-1} B (SPD):  Bbar = A^{-1} Xbar, Abar = -fold(Bbar X^T).
+ * recompute = 0: F is the factor L kept by the forward pass;
+ * recompute = 1: F is A and the factor is recomputed into Abar (for callers
+ * that do not keep L: no memory beyond the outputs).  info as for potrf. */
+int blast_hip_dposv_backward_batched(void* stream, int recompute, const blast_matrix_t* F, const blast_matrix_t* X,
+                                     const blast_matrix_t* Xbar, const blast_matrix_t* Abar,
+                                     const blast_matrix_t* Bbar, int64_t batch, int32_t* info);
+int blast_hip_sposv_backward_batched(void* stream, int recompute, const blast_matrix_t* F, const blast_matrix_t* X,
+                                     const blast_matrix_t* Xbar, const blast_matrix_t* Abar,
+                                     const blast_matrix_t* Bbar, int64_t batch, int32_t* info);
+
+/* X = A^{-1} B from getrf's factors (LU in place, ipiv as written by
+ * blast_hip_?getrf_batched): X = U^{-1} L^{-1} P B (LAPACK getrs). */
+int blast_hip_dgetrs_batched(void* stream, const blast_matrix_t* LU, const int64_t* ipiv, int64_t ipiv_stride,
+                             const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch);
+int blast_hip_sgetrs_batched(void* stream, const blast_matrix_t* LU, const int64_t* ipiv, int64_t ipiv_stride,
+                             const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch);
+
+/* getrs adjoint:  Bbar = A^{-T} Xbar = P^T L^{-T} U^{-T} Xbar,
+ * Abar = -Bbar X^T (Abar may be NULL). */
+int blast_hip_dgetrs_backward_batched(void* stream, const blast_matrix_t* LU, const int64_t* ipiv,
+                                      int64_t ipiv_stride, const blast_matrix_t* X, const blast_matrix_t* Xbar,
+                                      const blast_matrix_t* Abar, const blast_matrix_t* Bbar, int64_t batch);
+int blast_hip_sgetrs_backward_batched(void* stream, const blast_matrix_t* LU, const int64_t* ipiv,
+                                      int64_t ipiv_stride, const blast_matrix_t* X, const blast_matrix_t* Xbar,
+                                      const blast_matrix_t* Abar, const blast_matrix_t* Bbar, int64_t batch);
+
 #ifdef __cplusplus
 }
 #endif
