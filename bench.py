@@ -45,6 +45,9 @@ def parse():
     p.add_argument("--batch", type=int, default=BATCH)
     p.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--riccati-count", type=int, default=20000, help="instances of the uneven Riccati workload")
+    p.add_argument("--only", choices=("all", "posv", "gemm", "riccati"), default="all",
+                   help="run one workload alone (diagnostics; the bench line needs 'all')")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="first run a 256 MiB device copy (known HBM bytes) to scale rocprofv3 FETCH/WRITE_SIZE")
     return p.parse_args()
@@ -157,6 +160,96 @@ def gemm_workload(steps, warmup, batch=4096, n=64):
             "kernel": "gemm_tile_kernel<double, 0>"}
 
 
+def riccati_inputs(count=20000, horizon=10, seed=20240611):
+    """Uneven Riccati batch: nx ~ U{4..200}, nu ~ U{1..nx/2}, A, B ~ N(0, 1/nx),
+    Q = I + 0.1 W W^T (W ~ N(0,1)), R = I, generated on the GPU per nx group."""
+    import torch
+
+    from blast_amd import capi
+
+    rng = np.random.default_rng(seed)
+    nx = rng.integers(4, 201, size=count).astype(np.int32)
+    nu = rng.integers(1, np.maximum(1, nx // 2) + 1).astype(np.int32)
+    batch = capi.RiccatiBatch(nx, nu, horizon)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    inst = batch.inst
+    buf = batch.input
+    for n in np.unique(nx):
+        sel = np.nonzero(nx == n)[0]
+        g, n = len(sel), int(n)
+        m_max = max(1, n // 2)
+        m = torch.from_numpy(nu[sel].astype(np.int64)).cuda()
+        cols_nn = torch.arange(n * n, device="cuda")
+        off = {k: torch.from_numpy(inst[k][sel]).cuda() for k in ("a", "b", "q", "r")}
+        a = torch.randn(g, n, n, dtype=torch.float64, device="cuda", generator=gen) / np.sqrt(n)
+        buf[(off["a"][:, None] + cols_nn[None]).reshape(-1)] = a.transpose(1, 2).reshape(-1)
+        w = torch.randn(g, n, n, dtype=torch.float64, device="cuda", generator=gen)
+        q = torch.eye(n, dtype=torch.float64, device="cuda") + 0.1 * torch.bmm(w, w.transpose(1, 2).contiguous())
+        buf[(off["q"][:, None] + cols_nn[None]).reshape(-1)] = q.transpose(1, 2).reshape(-1)
+        bm = torch.randn(g, m_max, n, dtype=torch.float64, device="cuda", generator=gen) / np.sqrt(n)
+        flat = torch.arange(m_max * n, device="cuda")
+        keep = (flat[None] // n) < m[:, None]  # column j = flat // n of B kept when j < nu
+        idx = off["b"][:, None] + flat[None]
+        buf[idx[keep]] = bm.reshape(g, -1)[keep]
+        diag = torch.arange(m_max, device="cuda")
+        keep = diag[None] < m[:, None]
+        ridx = off["r"][:, None] + diag[None] * (m[:, None] + 1)
+        buf[ridx[keep]] = 1.0
+    torch.cuda.synchronize()
+    return batch
+
+
+def riccati_workload(steps, warmup, count=20000, horizon=10):
+    """One persistent launch over the whole uneven batch, eager and replayed
+    from a captured hipGraph; validated on a sample against the FP64 recursion."""
+    import torch
+
+    from blast_amd import capi
+    from oracle import riccati
+
+    batch = riccati_inputs(count, horizon)
+    nx, nu = batch.inst["nx"].astype(np.float64), batch.inst["nu"].astype(np.float64)
+    flops = float(np.sum(horizon * riccati.step_flops(nx, nu) + nx ** 3 / 3))
+    io_bytes = 8 * float(np.sum(2 * nx * nx + nx * nu + nu * nu + horizon * nu * nx + nx * nx))
+    ms = batch.autotune(reps=2)
+    t_eager = time_steps(batch.run, steps, warmup)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        batch.run(stream=s.cuda_stream)
+        s.synchronize()
+        with torch.cuda.graph(graph, stream=s):
+            batch.run(stream=s.cuda_stream)
+    torch.cuda.current_stream().wait_stream(s)
+    batch.output.zero_()
+    t_graph = time_steps(graph.replay, steps, warmup)
+    # validate a sample of instances (all sizes represented) against the oracle
+    host_in = batch.input.cpu().numpy()
+    K, L0 = batch.results()
+    info = batch.info.cpu().numpy()
+    worst = 0.0
+    for i in np.linspace(0, count - 1, 24).astype(int):
+        d = batch.inst[i]
+        x, u = int(d["nx"]), int(d["nu"])
+        get = lambda k, r, c: host_in[d[k]:d[k] + r * c].reshape(c, r).T  # noqa: E731
+        Kr, Pr = riccati.riccati(get("a", x, x), get("b", x, u), get("q", x, x), get("r", u, u), horizon)
+        worst = max(worst, np.abs(K[i] - Kr).max() / np.abs(Kr).max(),
+                    np.abs(L0[i] @ L0[i].T - Pr).max() / np.abs(Pr).max())
+    return {
+        "instances": count, "horizon": horizon,
+        "instances_per_s_graph": count / t_graph, "instances_per_s_eager": count / t_eager,
+        "ms_per_step_graph": t_graph * 1e3, "ms_per_step_eager": t_eager * 1e3,
+        "gflops": flops / t_graph / 1e9, "fp64_frac_of_peak": flops / t_graph / 1e12 / PEAK_FP64_TFLOPS,
+        "io_gbs": io_bytes / t_graph / 1e9,
+        "max_rel_err_sample_vs_fp64_recursion": float(worst),
+        "info_nonzero": int((info[:count] != 0).sum()),
+        "tuning_strip_width": capi.riccati_get_tuning().tolist(),
+        "autotune_ms": {str(k): [round(float(v), 4) for v in ms[k]] for k in range(16) if ms[k].any()},
+        "kernel": "riccati_persistent_kernel",
+    }
+
+
 def cpu_baseline(la, lb, abuf, bbuf, target_s):
     from oracle import oracle
 
@@ -169,7 +262,7 @@ def cpu_baseline(la, lb, abuf, bbuf, target_s):
     _, used = oracle.posv_mt(la, a, la, l, lb, b, lb, x, sample, threads=cores)
     dt = time.perf_counter() - t0
     reps = max(1, int(target_s / max(dt, 1e-6)))
-    reps = min(reps, 200)
+    reps = min(reps, 5000)
     a_big, b_big = np.tile(a, reps), np.tile(b, reps)
     l_big, x_big = np.zeros_like(a_big), np.zeros_like(b_big)
     t0 = time.perf_counter()
@@ -195,6 +288,11 @@ def main():
 
     from blast_amd import capi
 
+    if args.only in ("gemm", "riccati"):
+        res = (gemm_workload(args.steps, args.warmup) if args.only == "gemm"
+               else riccati_workload(args.steps, args.warmup, args.riccati_count))
+        print(json.dumps({args.only: res}))
+        return
     batch = args.batch
     if args.calibrate_copy:
         src = torch.ones(256 * 2 ** 20 // 8, dtype=torch.float64, device="cuda")
@@ -294,7 +392,10 @@ def main():
             "fp64_frac_of_peak": FLOPS_PER_INSTANCE * batch / t_plain / 1e12 / PEAK_FP64_TFLOPS,
         },
     }
-    line["config"]["other_workloads"] = {"dgemm_64_batch4096": gemm_workload(args.steps, args.warmup)}
+    line["config"]["other_workloads"] = {
+        "dgemm_64_batch4096": gemm_workload(args.steps, args.warmup),
+        f"riccati_uneven_{args.riccati_count}_h10": riccati_workload(args.steps, args.warmup, args.riccati_count),
+    }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(la, lb, abuf, bbuf, args.cpu_seconds)
     print(json.dumps(line))

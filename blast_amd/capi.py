@@ -72,6 +72,13 @@ def lib():
             getattr(L, f"blast_hip_{p}posv_backward_batched").argtypes = [vp, c_int, P, P, P, P, P, i64, i32p]
             getattr(L, f"blast_hip_{p}getrs_batched").argtypes = [vp, P, vp, i64, P, P, i64]
             getattr(L, f"blast_hip_{p}getrs_backward_batched").argtypes = [vp, P, vp, i64, P, P, P, P, i64]
+        L.blast_hip_riccati_plan.argtypes = [i64, vp, vp, c_int, vp, vp, vp, vp, vp]
+        L.blast_hip_driccati_batched.argtypes = [vp, c_int, vp, i64, vp, vp, vp, vp, vp]
+        L.blast_hip_riccati_set_tuning.argtypes = [vp]
+        L.blast_hip_riccati_get_tuning.argtypes = [vp]
+        L.blast_hip_riccati_autotune.argtypes = [vp, c_int, vp, vp, i64, vp, vp, vp, vp, vp, c_int, vp]
+        L.blast_hip_save_tuning.argtypes = [ctypes.c_char_p]
+        L.blast_hip_load_tuning.argtypes = [ctypes.c_char_p]
         L.blast_hip_set_algorithm.argtypes = [c_int]
         L.blast_hip_last_error.restype = ctypes.c_char_p
         L.blast_hip_version.restype = ctypes.c_char_p
@@ -237,3 +244,101 @@ def getrs_backward(LU, ipiv, ipiv_stride, X, Xb, Ab, Bb, batch, stream=None):
     name = f"blast_hip_{_prefix(X.layout.dtype)}getrs_backward_batched"
     return check(name, getattr(lib(), name)(_stream(stream), _ref(LU), ipiv.data_ptr(), ipiv_stride, _ref(X),
                                             _ref(Xb), _oref(Ab), _ref(Bb), batch))
+
+
+# ---- uneven batched Riccati recursion -------------------------------------
+
+RICCATI_INSTANCE = np.dtype([("nx", np.int32), ("nu", np.int32), ("a", np.int64), ("b", np.int64),
+                             ("q", np.int64), ("r", np.int64), ("k", np.int64), ("l", np.int64),
+                             ("work", np.int64)], align=True)
+
+
+def riccati_plan(nx, nu, horizon):
+    """Host plan (blast_hip_riccati_plan): descriptors, queue order and the
+    element counts of the input, output and work buffers."""
+    nx = np.ascontiguousarray(nx, dtype=np.int32)
+    nu = np.ascontiguousarray(nu, dtype=np.int32)
+    count = len(nx)
+    inst = np.zeros(count, dtype=RICCATI_INSTANCE)
+    order = np.zeros(count, dtype=np.int32)
+    sizes = np.zeros(3, dtype=np.int64)
+    check("blast_hip_riccati_plan",
+          lib().blast_hip_riccati_plan(count, nx.ctypes.data, nu.ctypes.data, int(horizon), inst.ctypes.data,
+                                       order.ctypes.data, sizes[0:].ctypes.data, sizes[1:].ctypes.data,
+                                       sizes[2:].ctypes.data))
+    return inst, order, [int(v) for v in sizes]
+
+
+class RiccatiBatch:
+    """Device buffers of one uneven Riccati batch, laid out by riccati_plan."""
+
+    def __init__(self, nx, nu, horizon, device="cuda"):
+        import torch
+
+        self.horizon = int(horizon)
+        self.inst, self.order, (n_in, n_out, n_work) = riccati_plan(nx, nu, horizon)
+        self.count = len(self.inst)
+        self.d_inst = torch.from_numpy(self.inst.view(np.uint8).copy()).to(device)
+        queue = np.concatenate([np.zeros(1, np.int32), self.order])
+        self.queue = torch.from_numpy(queue).to(device)
+        self.input = torch.zeros(max(n_in, 1), dtype=torch.float64, device=device)
+        self.output = torch.zeros(max(n_out, 1), dtype=torch.float64, device=device)
+        self.work = torch.zeros(max(n_work, 1), dtype=torch.float64, device=device)
+        self.info = torch.full((max(self.count, 1),), -1, dtype=torch.int32, device=device)
+
+    def pack(self, A, B, Q, R):
+        """Host lists of per-instance matrices -> the device input buffer."""
+        import torch
+
+        host = np.zeros(self.input.numel(), dtype=np.float64)
+        for i, d in enumerate(self.inst):
+            for key, m in (("a", A[i]), ("b", B[i]), ("q", Q[i]), ("r", R[i])):
+                m = np.asarray(m, dtype=np.float64)
+                host[d[key]:d[key] + m.size] = m.ravel(order="F")
+        self.input.copy_(torch.from_numpy(host))
+
+    def run(self, stream=None):
+        return check("blast_hip_driccati_batched",
+                     lib().blast_hip_driccati_batched(_stream(stream), self.horizon, self.d_inst.data_ptr(),
+                                                      self.count, self.input.data_ptr(), self.output.data_ptr(),
+                                                      self.work.data_ptr(), self.queue.data_ptr(),
+                                                      self.info.data_ptr()))
+
+    def autotune(self, reps=3, stream=None):
+        ms = np.zeros(48, dtype=np.float32)
+        check("blast_hip_riccati_autotune",
+              lib().blast_hip_riccati_autotune(_stream(stream), self.horizon, self.d_inst.data_ptr(),
+                                               self.inst.ctypes.data, self.count, self.input.data_ptr(),
+                                               self.output.data_ptr(), self.work.data_ptr(), self.queue.data_ptr(),
+                                               self.order.ctypes.data, int(reps), ms.ctypes.data))
+        return ms.reshape(16, 3)
+
+    def results(self):
+        """Host copies: list of K (horizon x nu x nx) and list of L0 (nx x nx)."""
+        out = self.output.cpu().numpy()
+        K, L0 = [], []
+        for d in self.inst:
+            nx, nu = int(d["nx"]), int(d["nu"])
+            k = out[d["k"]:d["k"] + self.horizon * nu * nx].reshape(self.horizon, nx, nu).transpose(0, 2, 1)
+            K.append(k)
+            L0.append(out[d["l"]:d["l"] + nx * nx].reshape(nx, nx).T)
+        return K, L0
+
+
+def riccati_get_tuning():
+    t = np.zeros(16, dtype=np.int8)
+    check("blast_hip_riccati_get_tuning", lib().blast_hip_riccati_get_tuning(t.ctypes.data))
+    return t
+
+
+def riccati_set_tuning(t):
+    t = np.ascontiguousarray(t, dtype=np.int8)
+    return check("blast_hip_riccati_set_tuning", lib().blast_hip_riccati_set_tuning(t.ctypes.data))
+
+
+def save_tuning(path):
+    return check("blast_hip_save_tuning", lib().blast_hip_save_tuning(os.fsencode(path)))
+
+
+def load_tuning(path):
+    return check("blast_hip_load_tuning", lib().blast_hip_load_tuning(os.fsencode(path)))

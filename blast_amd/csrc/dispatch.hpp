@@ -50,6 +50,14 @@ int launch_fill_i32(hipStream_t, int32_t* p, int32_t v, int64_t count);
 int dispatch_dsposv(hipStream_t, int bf16, int itmax, double tol, Mat<double> A, Mat<double> B, Mat<double> X,
                     int64_t batch, int32_t* info, int32_t* iter);
 
+// riccati.hip: trailing-update strip width (MFMA tiles per wavefront) per
+// nx bucket (nx / 16), passed by value.
+struct RiccatiTuning {
+    int8_t tn[16];
+};
+int launch_riccati(hipStream_t, int horizon, const void* inst, int64_t start, int64_t count, const double* in,
+                   double* out, double* work, int32_t* queue, RiccatiTuning const& tuning, int32_t* info, int grid);
+
 // dispatch.cpp
 template <typename T> int dispatch_gemm(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t);
 template <typename T> int dispatch_syrk(hipStream_t, T, Mat<T>, T, Mat<T>, Mat<T>, int64_t);

@@ -1,0 +1,262 @@
+// Uneven batched Riccati recursion (LQR / MPC backward pass), FP64, one
+// persistent launch.
+//
+// Per instance (nx states, nu inputs, horizon N; A nx x nx, B nx x nu,
+// Q nx x nx SPD, R nu x nu SPD; dense column-major, leading dimension =
+// rows), with P_N = Q and L = chol(P):
+//   M   = L^T [B A]                         gemm (triangular operand)
+//   Lam = blkdiag(R, Q) + M^T M             syrk, lower
+//   chol(Lam) = [Ls 0; Lks Lp]              potrf (blocked, MFMA trailing update)
+//   K_k = -Ls^{-T} Lks^T                    trsm
+//   L  <- Lp   (P_k = Q + A^T P A - H^T S^{-1} H, S = R + B^T P B, H = B^T P A)
+// for k = N-1 .. 0, i.e. the square-root form of
+//   K_k = -(R + B^T P B)^{-1} B^T P A,  P <- Q + A^T P A + A^T P B K_k.
+// Outputs: K_k (nu x nx, k = 0..N-1) and L_0 = chol(P_0) (lower).
+// This composite is not in BLAST; it strings together the entry points BLAST
+// provides (gemm, syrkLower, potrf, trsm) the way the reference's
+// syrk+potrf benchmarks and BLASFEO wrapper do (bench/blas/SyrkPotrf.cpp,
+// include/blasfeo/blasfeo_api/SyrkPotrf.hpp) and is checked against the
+// textbook recursion in FP64.
+//
+// Scheduling: workgroups pull instance indices from queue[1..count] through
+// the counter queue[0] (atomic); the host orders the queue by size bucket,
+// largest first, so the tail of the launch is made of small instances.
+// The trailing-update strip width (1, 2 or 4 MFMA tiles per wavefront) is
+// taken per nx-bucket from the tuning table.
+
+#include "dispatch.hpp"
+#include "launch.hpp"
+#include "tiles.hpp"
+
+namespace blast_hip {
+
+using RiccatiInstance = blast_riccati_instance_t;
+
+namespace {
+
+constexpr int kT = 256;
+
+// Dense column-major view: element (i, j) at p[i + ld*j] (or p[j + ld*i] transposed).
+struct DV {
+    double* p;
+    int ld;
+    bool t;
+    __device__ double& operator()(int i, int j) const { return t ? p[j + int64_t(ld) * i] : p[i + int64_t(ld) * j]; }
+    __device__ DV sub(int i, int j) const { return DV{&(*this)(i, j), ld, t}; }
+    __device__ DV tr() const { return DV{p, ld, !t}; }
+};
+
+enum { kPlain = 0, kSyrkAdd = 1, kUpperA = 2 };
+
+// D = alpha*op(A)*op(B) + beta*C on 16x16 MFMA tiles, whole workgroup,
+// TN tiles per wavefront strip.  kSyrkAdd: lower triangle of an M x M
+// result only; kUpperA: A masked to its upper triangle.
+template <int TN, int MODE>
+__device__ void wg_gemm(double alpha, DV A, DV B, double beta, DV C, DV D, int M, int N, int K) {
+    using Mf = Mfma<double>;
+    int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int const m = lane & 15, g = lane >> 4;
+    int const MT = (M + 15) / 16, NTt = (N + 15) / 16, KQ = (K + 3) / 4;
+    for (int R = w; R < MT; R += kT / 64) {
+        int const row = 16 * R + m;
+        for (int T0 = 0; T0 < NTt; T0 += TN) {
+            if (MODE == kSyrkAdd && 16 * T0 > 16 * R + 15) break;
+            d4 acc[TN];
+#pragma unroll
+            for (int t = 0; t < TN; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+            for (int q = 0; q < KQ; ++q) {
+                int const k = 4 * q + g;
+                bool const ina = row < M && k < K && (MODE != kUpperA || k >= row);
+                double const av = ina ? A(row, k) : 0.0;
+#pragma unroll
+                for (int t = 0; t < TN; ++t) {
+                    int const col = 16 * (T0 + t) + m;
+                    double const bv = (col < N && k < K) ? B(k, col) : 0.0;
+                    acc[t] = Mf::mma(av, bv, acc[t]);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < TN; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    int const i = 16 * R + Mf::row(lane, r), j = 16 * (T0 + t) + acc_col(lane);
+                    if (i < M && j < N && (MODE != kSyrkAdd || i >= j))
+                        D(i, j) = beta == 0.0 ? alpha * acc[t][r] : __builtin_fma(beta, C(i, j), alpha * acc[t][r]);
+                }
+        }
+    }
+    __syncthreads();
+}
+
+// In-place lower Cholesky of the n x n matrix a (lower triangle read and
+// written), blocked by 16 columns: the 16x16 diagonal block is factored in
+// LDS, the panel below it solved row by row, the trailing matrix updated on
+// MFMA tiles.  Returns the first non-positive pivot column (1-based) or 0.
+template <int TN>
+__device__ int wg_potrf(DV a, int n, double* sB) {
+    int bad = 0;
+    for (int k0 = 0; k0 < n; k0 += 16) {
+        int const kb = min(16, n - k0);
+        // diagonal block -> LDS (17-stride), factor, write back
+        for (int e = threadIdx.x; e < kb * kb; e += kT) {
+            int const i = e % kb, j = e / kb;
+            sB[j * 17 + i] = i >= j ? a(k0 + i, k0 + j) : 0.0;
+        }
+        __syncthreads();
+        for (int c = 0; c < kb; ++c) {
+            double const d = sB[c * 17 + c];
+            if (!(d > 0.0) && bad == 0) bad = k0 + c + 1;
+            double const s = __builtin_sqrt(d);
+            __syncthreads();
+            if (threadIdx.x < kb && int(threadIdx.x) >= c) sB[c * 17 + threadIdx.x] = int(threadIdx.x) == c ? s : sB[c * 17 + threadIdx.x] / s;
+            __syncthreads();
+            for (int e = threadIdx.x; e < kb * kb; e += kT) {
+                int const i = e % kb, j = e / kb;
+                if (j > c && i >= j) sB[j * 17 + i] = __builtin_fma(-sB[c * 17 + i], sB[c * 17 + j], sB[j * 17 + i]);
+            }
+            __syncthreads();
+        }
+        for (int e = threadIdx.x; e < kb * kb; e += kT) {
+            int const i = e % kb, j = e / kb;
+            if (i >= j) a(k0 + i, k0 + j) = sB[j * 17 + i];
+        }
+        // panel: rows below solve x L_kk^T = a_row
+        for (int i = k0 + kb + threadIdx.x; i < n; i += kT) {
+            double x[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                if (c < kb) {
+                    double v = a(i, k0 + c);
+#pragma unroll
+                    for (int j = 0; j < 16; ++j)
+                        if (j < c) v = __builtin_fma(-x[j], sB[j * 17 + c], v);
+                    x[c] = v / sB[c * 17 + c];
+                    a(i, k0 + c) = x[c];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        __syncthreads();
+        // trailing update, lower triangle: A22 -= L21 L21^T
+        int const rest = n - k0 - kb;
+        if (rest > 0) {
+            DV const l21 = a.sub(k0 + kb, k0), a22 = a.sub(k0 + kb, k0 + kb);
+            wg_gemm<TN, kSyrkAdd>(-1.0, l21, l21.tr(), 1.0, a22, a22, rest, rest, kb);
+        }
+    }
+    return bad;
+}
+
+// Solve U X = B in place for upper-triangular U (here U = Ls^T as a
+// transposed view), n x nrhs, by a column sweep from the bottom.
+__device__ void wg_trsm_upper(DV U, DV Bm, int n, int nrhs) {
+    for (int k = n - 1; k >= 0; --k) {
+        double const ukk = U(k, k);
+        for (int j = threadIdx.x; j < nrhs; j += kT) Bm(k, j) = Bm(k, j) / ukk;
+        __syncthreads();
+        for (int e = threadIdx.x; e < k * nrhs; e += kT) {
+            int const i = e % k, j = e / k;
+            Bm(i, j) = __builtin_fma(-U(i, k), Bm(k, j), Bm(i, j));
+        }
+        __syncthreads();
+    }
+}
+
+template <int TN>
+__device__ void riccati_instance(RiccatiInstance const& ins, int horizon, const double* in, double* out, double* work,
+                                 double* sB, int32_t* info_out) {
+    int const nx = ins.nx, nu = ins.nu, nt = nx + nu;
+    DV const A{const_cast<double*>(in) + ins.a, nx, false};
+    DV const B{const_cast<double*>(in) + ins.b, nx, false};
+    DV const Q{const_cast<double*>(in) + ins.q, nx, false};
+    DV const R{const_cast<double*>(in) + ins.r, nu, false};
+    double* wp = work + ins.work;
+    DV const BA{wp, nx, false};                   // [B A], nx x nt
+    DV const Mm{wp + int64_t(nx) * nt, nx, false};  // M = L^T [B A], nx x nt
+    DV const Lam{wp + 2 * int64_t(nx) * nt, nt, false};
+    DV const Lcur = Lam.sub(nu, nu);              // chol(P) lives in Lam's trailing block
+    int bad = 0;
+    for (int e = threadIdx.x; e < nx * nt; e += kT) {
+        int const i = e % nx, j = e / nx;
+        BA(i, j) = j < nu ? B(i, j) : A(i, j - nu);
+    }
+    for (int e = threadIdx.x; e < nx * nx; e += kT) {
+        int const i = e % nx, j = e / nx;
+        Lcur(i, j) = i >= j ? Q(i, j) : 0.0;
+    }
+    __syncthreads();
+    int b0 = wg_potrf<TN>(Lcur, nx, sB);
+    if (b0) bad = b0;
+    for (int k = horizon - 1; k >= 0; --k) {
+        // M = L^T [B A]
+        wg_gemm<TN, kUpperA>(1.0, Lcur.tr(), BA, 0.0, Mm, Mm, nx, nt, nx);
+        // Lam = blkdiag(R, Q) (lower) + M^T M
+        for (int e = threadIdx.x; e < nt * nt; e += kT) {
+            int const i = e % nt, j = e / nt;
+            if (i >= j) Lam(i, j) = (i < nu && j < nu) ? R(i, j) : (i >= nu && j >= nu ? Q(i - nu, j - nu) : 0.0);
+        }
+        __syncthreads();
+        wg_gemm<TN, kSyrkAdd>(1.0, Mm.tr(), Mm, 1.0, Lam, Lam, nt, nt, nx);
+        int const bk = wg_potrf<TN>(Lam, nt, sB);
+        if (bk && bad == 0) bad = bk;
+        // K_k = -Ls^{-T} Lks^T
+        DV const Kk{out + ins.k + int64_t(k) * nu * nx, nu, false};
+        for (int e = threadIdx.x; e < nu * nx; e += kT) {
+            int const i = e % nu, j = e / nu;
+            Kk(i, j) = -Lam(nu + j, i);
+        }
+        __syncthreads();
+        wg_trsm_upper(Lam.tr(), Kk, nu, nx);
+    }
+    DV const L0{out + ins.l, nx, false};
+    for (int e = threadIdx.x; e < nx * nx; e += kT) {
+        int const i = e % nx, j = e / nx;
+        L0(i, j) = i >= j ? Lcur(i, j) : 0.0;
+    }
+    if (threadIdx.x == 0 && info_out) *info_out = bad;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kT) void riccati_persistent_kernel(const RiccatiInstance* inst, int64_t start,
+                                                                int64_t count, int horizon, const double* in,
+                                                                double* out, double* work, int32_t* queue,
+                                                                RiccatiTuning tuning, int32_t* info) {
+    __shared__ double sB[16 * 17];
+    __shared__ int sIdx;
+    for (;;) {
+        if (threadIdx.x == 0) sIdx = atomicAdd(&queue[0], 1);
+        __syncthreads();
+        int const q = sIdx;
+        __syncthreads();
+        if (q >= count) break;  // every workgroup reaches this exit
+        int const idx = queue[1 + start + q];
+        RiccatiInstance const ins = inst[idx];
+        int const tn = tuning.tn[min(ins.nx / 16, 15)];
+        int32_t* inf = info ? info + idx : nullptr;
+        if (tn == 1)
+            riccati_instance<1>(ins, horizon, in, out, work, sB, inf);
+        else if (tn == 2)
+            riccati_instance<2>(ins, horizon, in, out, work, sB, inf);
+        else
+            riccati_instance<4>(ins, horizon, in, out, work, sB, inf);
+    }
+}
+
+__global__ void zero_counter_kernel(int32_t* p) { *p = 0; }
+
+}  // namespace
+
+int launch_riccati(hipStream_t s, int horizon, const void* inst, int64_t start, int64_t count, const double* in,
+                   double* out, double* work, int32_t* queue, RiccatiTuning const& tuning, int32_t* info, int grid) {
+    if (count == 0) return 0;
+    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, queue);
+    int r = check_launch("zero_counter_kernel");
+    if (r) return r;
+    int64_t const g = grid > 0 ? (grid < count ? grid : count) : (count < 2048 ? count : 2048);
+    hipLaunchKernelGGL(riccati_persistent_kernel, dim3(unsigned(g)), dim3(kT), 0, s,
+                       static_cast<const RiccatiInstance*>(inst), start, count, horizon, in, out, work, queue, tuning,
+                       info);
+    return check_launch("riccati_persistent_kernel");
+}
+
+}  // namespace blast_hip

@@ -210,6 +210,71 @@ int blast_hip_sgetrs_backward_batched(void* stream, const blast_matrix_t* LU, co
                                       int64_t ipiv_stride, const blast_matrix_t* X, const blast_matrix_t* Xbar,
                                       const blast_matrix_t* Abar, const blast_matrix_t* Bbar, int64_t batch);
 
+/* ---- Uneven batched Riccati recursion (LQR / MPC backward pass), FP64 ----
+ * No BLAST function is replaced: this composite strings together the entry
+ * points above (gemm -> syrkLower -> potrf -> trsm per step) the way the
+ * reference's syrk+potrf benchmarks and BLASFEO wrapper do
+ * (bench/blas/SyrkPotrf.cpp:15-57, include/blasfeo/blasfeo_api/SyrkPotrf.hpp:14-39).
+ *
+ * Instance i has nx states and nu inputs.  Its inputs A (nx x nx), B (nx x nu),
+ * Q (nx x nx, SPD) and R (nu x nu, SPD) are dense column-major with leading
+ * dimension = rows, at element offsets a, b, q, r of the input buffer `in`.
+ * Starting from P_N = Q, for k = N-1 .. 0:
+ *   K_k = -(R + B^T P B)^{-1} B^T P A,   P <- Q + A^T P A + A^T P B K_k,
+ * computed in square-root form on chol(P).  Outputs in `out`: K_k (nu x nx,
+ * column-major, k = 0..N-1 consecutive) at offset k and L_0 = chol(P_0)
+ * (nx x nx lower, upper triangle zeroed) at offset l.  `work` offsets
+ * index the FP64 scratch buffer (2*nx*(nx+nu) + (nx+nu)^2 elements each).
+ * Only Q's and R's lower triangles are read. */
+typedef struct blast_riccati_instance_t {
+    int32_t nx, nu;
+    int64_t a, b, q, r; /* element offsets into `in` */
+    int64_t k, l;       /* element offsets into `out` */
+    int64_t work;       /* element offset into `work` */
+} blast_riccati_instance_t;
+
+/* Host-side plan: fills inst[count] with consecutive offsets, each rounded
+ * up to 8 elements (inputs A, B, Q, R per instance in that order; outputs K
+ * then L), the queue order (order[],
+ * instances sorted by nx bucket nx/16 descending, then by (nx+nu)^3
+ * descending, so the tail of a launch holds the small instances), and the
+ * total element counts of `in`, `out` and `work`.  1 <= nx, nu <= 4096. */
+int blast_hip_riccati_plan(int64_t count, const int32_t* nx, const int32_t* nu, int horizon,
+                           blast_riccati_instance_t* inst, int32_t* order, int64_t* in_elems, int64_t* out_elems,
+                           int64_t* work_elems);
+
+/* One persistent launch over all `count` instances.  inst: DEVICE array of
+ * count descriptors.  queue: DEVICE int32 array of count + 1 entries whose
+ * entries 1..count hold the queue order (a permutation of 0..count-1, e.g.
+ * the plan's order); entry 0 is the work counter, reset by a kernel at the
+ * start of every call (no memset, so the call can be captured in a hipGraph
+ * and replayed).  info (optional DEVICE int32[count]): 0, or the first
+ * column (1-based) of a non-positive pivot met in any step's potrf.  Calls
+ * that share `queue` or `work` must not overlap. */
+int blast_hip_driccati_batched(void* stream, int horizon, const blast_riccati_instance_t* inst, int64_t count,
+                               const double* in, double* out, double* work, int32_t* queue, int32_t* info);
+
+/* Strip-width tuning table (MFMA 16x16 tiles per wavefront in the trailing
+ * updates, 1, 2 or 4) indexed by nx bucket min(nx/16, 15); process-wide. */
+int blast_hip_riccati_set_tuning(const int8_t* strip_width16);
+int blast_hip_riccati_get_tuning(int8_t* strip_width16);
+
+/* Autotuner: for every nx bucket of the queue order (order_host: the queue's
+ * entries 1..count, on the host; inst_host: the descriptors, on the host)
+ * times one launch over that bucket's instances with each strip width,
+ * `reps` times after a warm-up, and keeps the fastest in the table.
+ * Synchronises the stream.  ms_out (optional, host float[48]) receives the
+ * average time per bucket and width (index bucket*3 + {0: 1, 1: 2, 2: 4}). */
+int blast_hip_riccati_autotune(void* stream, int horizon, const blast_riccati_instance_t* inst,
+                               const blast_riccati_instance_t* inst_host, int64_t count, const double* in,
+                               double* out, double* work, int32_t* queue, const int32_t* order_host, int reps,
+                               float* ms_out);
+
+/* Persist / restore the tuning table (text file, first line names the
+ * format and target architecture; a file for another target is rejected). */
+int blast_hip_save_tuning(const char* path);
+int blast_hip_load_tuning(const char* path);
+
 #ifdef __cplusplus
 }
 #endif
