@@ -72,6 +72,12 @@ def lib():
             getattr(L, f"blast_hip_{p}posv_backward_batched").argtypes = [vp, c_int, P, P, P, P, P, i64, i32p]
             getattr(L, f"blast_hip_{p}getrs_batched").argtypes = [vp, P, vp, i64, P, P, i64]
             getattr(L, f"blast_hip_{p}getrs_backward_batched").argtypes = [vp, P, vp, i64, P, P, P, P, i64]
+        L.blast_hip_mixed_work_elems.argtypes = [i64]
+        L.blast_hip_mixed_work_elems.restype = i64
+        L.blast_hip_dsposv_ws_batched.argtypes = [vp, c_int, c_int, ctypes.c_double, P, P, P, vp, i64, i32p, i32p]
+        L.blast_hip_dgesv_batched.argtypes = [vp, P, vp, i64, P, P, i64, i32p]
+        L.blast_hip_dsgesv_batched.argtypes = [vp, c_int, c_int, ctypes.c_double, P, vp, i64, P, P, vp, i64, i32p,
+                                               i32p]
         L.blast_hip_riccati_plan.argtypes = [i64, vp, vp, c_int, vp, vp, vp, vp, vp]
         L.blast_hip_driccati_batched.argtypes = [vp, c_int, vp, i64, vp, vp, vp, vp, vp]
         L.blast_hip_riccati_set_tuning.argtypes = [vp]
@@ -205,6 +211,40 @@ def sposv(precision, A, B, X, batch, iter_, info=None, itmax=30, tol=0.0, stream
     name = "blast_hip_dsposv_batched"
     return check(name, lib().blast_hip_dsposv_batched(_stream(stream), int(precision), int(itmax), float(tol), _ref(A),
                                                       _ref(B), _ref(X), batch, _info_ptr(info), _info_ptr(iter_)))
+
+
+def mixed_work_elems(n):
+    """FP32 factor buffer (floats per instance) of sposv_ws / sgesv."""
+    return int(lib().blast_hip_mixed_work_elems(int(n)))
+
+
+def mixed_work(n, batch, device="cuda"):
+    import torch
+
+    return torch.empty(max(1, mixed_work_elems(n) * batch), dtype=torch.float32, device=device)
+
+
+def sposv_ws(precision, A, B, X, work, batch, iter_, info=None, itmax=30, tol=0.0, stream=None):
+    """Mixed precision Cholesky solve, n <= 256, nrhs <= 8 (LAPACK dsposv)."""
+    name = "blast_hip_dsposv_ws_batched"
+    return check(name, lib().blast_hip_dsposv_ws_batched(_stream(stream), int(precision), int(itmax), float(tol),
+                                                         _ref(A), _ref(B), _ref(X), _info_ptr(work), batch,
+                                                         _info_ptr(info), _info_ptr(iter_)))
+
+
+def gesv(A, ipiv, ipiv_stride, B, X, batch, info=None, stream=None):
+    """FP64 LU with partial pivoting in place of A, then X = A^{-1} B (LAPACK dgesv)."""
+    name = "blast_hip_dgesv_batched"
+    return check(name, lib().blast_hip_dgesv_batched(_stream(stream), _ref(A), _info_ptr(ipiv), ipiv_stride, _ref(B),
+                                                     _ref(X), batch, _info_ptr(info)))
+
+
+def sgesv(precision, A, ipiv, ipiv_stride, B, X, work, batch, iter_, info=None, itmax=30, tol=0.0, stream=None):
+    """Mixed precision LU + FP64 refinement (LAPACK dsgesv)."""
+    name = "blast_hip_dsgesv_batched"
+    return check(name, lib().blast_hip_dsgesv_batched(_stream(stream), int(precision), int(itmax), float(tol),
+                                                      _ref(A), _info_ptr(ipiv), ipiv_stride, _ref(B), _ref(X),
+                                                      _info_ptr(work), batch, _info_ptr(info), _info_ptr(iter_)))
 
 
 def _oref(op):

@@ -310,6 +310,25 @@ int sposv_entry(void* stream, int precision, int itmax, double tol, const blast_
                            Mat<double>::from(*X), batch, info, iter);
 }
 
+// Shared checks of the mixed-precision and LU solve entry points.
+int solve_shapes(const char* who, const blast_matrix_t* A, const blast_matrix_t* B, const blast_matrix_t* X, int ia,
+                 int ib, int ix) {
+    if (!valid<double>(A, "A")) return -ia;
+    if (!valid<double>(B, "B")) return -ib;
+    if (!valid<double>(X, "X")) return -ix;
+    if (A->rows != A->cols) { set_error("%s: A must be square", who); return -ia; }
+    if (B->rows != A->rows) { set_error("%s: Invalid argument size (B)", who); return -ib; }
+    if (X->rows != B->rows || X->cols != B->cols) { set_error("%s: Invalid argument size (X)", who); return -ix; }
+    return 0;
+}
+
+int mixed_params(const char* who, int precision, int itmax, double tol) {
+    if (precision != 0 && precision != 1) { set_error("%s: precision must be 0 (FP32) or 1 (BF16)", who); return -2; }
+    if (itmax < 0 || itmax > 1000) { set_error("%s: itmax out of range", who); return -3; }
+    if (!(tol >= 0.0 && tol < 1.0)) { set_error("%s: tol must be in [0, 1)", who); return -4; }
+    return 0;
+}
+
 }  // namespace
 }  // namespace blast_hip
 
@@ -383,6 +402,55 @@ int blast_hip_dsposv_batched(void* s, int precision, int itmax, double tol, cons
                              const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch, int32_t* info,
                              int32_t* iter) {
     return sposv_entry(s, precision, itmax, tol, A, B, X, batch, info, iter);
+}
+
+int64_t blast_hip_mixed_work_elems(int64_t n) { return (n >= 1 && n <= 256) ? wg_work_elems(int(n)) : 0; }
+
+int blast_hip_dsposv_ws_batched(void* s, int precision, int itmax, double tol, const blast_matrix_t* A,
+                                const blast_matrix_t* B, const blast_matrix_t* X, float* work, int64_t batch,
+                                int32_t* info, int32_t* iter) {
+    if (int r = mixed_params("dsposv", precision, itmax, tol)) return r;
+    if (int r = solve_shapes("dsposv", A, B, X, 5, 6, 7)) return r;
+    if (!valid_batch(batch)) return -9;
+    if (batch > 0 && !iter) { set_error("dsposv: iter is required"); return -11; }
+    if (batch > 0 && !work && wg_covers(int(A->rows), int(X->cols)) && !(A->rows <= 128 && X->cols <= 4)) {
+        set_error("dsposv: work is required for n > 128 or nrhs > 4");
+        return -8;
+    }
+    (void)hipGetLastError();
+    return dispatch_dsposv_ws(hipStream_t(s), precision, itmax, tol, Mat<double>::from(*A), Mat<double>::from(*B),
+                              Mat<double>::from(*X), work, batch, info, iter);
+}
+
+int blast_hip_dgesv_batched(void* s, const blast_matrix_t* A, int64_t* ipiv, int64_t ipiv_stride,
+                            const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch, int32_t* info) {
+    if (int r = solve_shapes("dgesv", A, B, X, 2, 5, 6)) return r;
+    if (A->trans) { set_error("dgesv: A must be column-major (trans = 0)"); return -2; }
+    if (A->rows > 0 && !ipiv) { set_error("dgesv: null ipiv"); return -3; }
+    if (ipiv_stride < A->rows) { set_error("dgesv: ipiv_stride < n"); return -4; }
+    if (!valid_batch(batch)) return -7;
+    (void)hipGetLastError();
+    return dispatch_dgesv(hipStream_t(s), Mat<double>::from(*A), ipiv, ipiv_stride, Mat<double>::from(*B),
+                          Mat<double>::from(*X), batch, info);
+}
+
+int blast_hip_dsgesv_batched(void* s, int precision, int itmax, double tol, const blast_matrix_t* A, int64_t* ipiv,
+                             int64_t ipiv_stride, const blast_matrix_t* B, const blast_matrix_t* X, float* work,
+                             int64_t batch, int32_t* info, int32_t* iter) {
+    if (int r = mixed_params("dsgesv", precision, itmax, tol)) return r;
+    if (int r = solve_shapes("dsgesv", A, B, X, 5, 8, 9)) return r;
+    if (A->trans) { set_error("dsgesv: A must be column-major (trans = 0)"); return -5; }
+    if (A->rows > 0 && !ipiv) { set_error("dsgesv: null ipiv"); return -6; }
+    if (ipiv_stride < A->rows) { set_error("dsgesv: ipiv_stride < n"); return -7; }
+    if (!valid_batch(batch)) return -11;
+    if (batch > 0 && !iter) { set_error("dsgesv: iter is required"); return -13; }
+    if (batch > 0 && !work && wg_covers(int(A->rows), int(X->cols))) {
+        set_error("dsgesv: work is required");
+        return -10;
+    }
+    (void)hipGetLastError();
+    return dispatch_dsgesv(hipStream_t(s), precision, itmax, tol, Mat<double>::from(*A), ipiv, ipiv_stride,
+                           Mat<double>::from(*B), Mat<double>::from(*X), work, batch, info, iter);
 }
 
 #define BOTH(P, T)                                                                                                \

@@ -38,6 +38,7 @@ int dispatch_potrf(hipStream_t s, Mat<T> A, Mat<T> L, int64_t batch, int32_t* in
             Mat<double> none{};
             int const r = launch_posv_tiles(s, A, L, none, none, batch, info);
             if (r != 1) return r;
+            if (A.rows == A.cols && wg_covers(A.rows, 0)) return launch_wg_posv(s, A, L, none, none, 0, batch, info);
         }
     }
     return launch_potrf_ref<T>(s, A, L, batch, info);
@@ -45,6 +46,13 @@ int dispatch_potrf(hipStream_t s, Mat<T> A, Mat<T> L, int64_t batch, int32_t* in
 
 template <typename T>
 int dispatch_getrf(hipStream_t s, Mat<T> A, int64_t* ipiv, int64_t ipiv_stride, int64_t batch, int32_t* info) {
+    if constexpr (std::is_same_v<T, double>) {
+        // column-major storage (partial pivoting); the row-major form has no pivoting
+        if (algorithm() == BLAST_HIP_ALGO_AUTO && !A.trans && A.rows == A.cols && wg_covers(A.rows, 0)) {
+            Mat<double> none{};
+            return launch_wg_gesv(s, A, ipiv, ipiv_stride, none, none, 0, batch, info);
+        }
+    }
     return launch_getrf_ref<T>(s, A, ipiv, ipiv_stride, batch, info);
 }
 
@@ -54,6 +62,7 @@ int dispatch_posv(hipStream_t s, Mat<T> A, Mat<T> L, Mat<T> B, Mat<T> X, int64_t
         if (algorithm() == BLAST_HIP_ALGO_AUTO && X.cols > 0) {
             int const r = launch_posv_tiles(s, A, L, B, X, batch, info);
             if (r != 1) return r;
+            if (wg_covers(A.rows, X.cols)) return launch_wg_posv(s, A, L, B, X, X.cols, batch, info);
         }
     }
     int r = launch_potrf_ref<T>(s, A, L, batch, info);
@@ -74,6 +83,46 @@ int dispatch_dsposv(hipStream_t s, int bf16, int itmax, double tol, Mat<double> 
     r = launch_fill_i32(s, iter, -3, batch);
     if (r) return r;
     return dispatch_posv<double>(s, A, A, B, X, batch, info);
+}
+
+// As dispatch_dsposv, for n <= 256 and nrhs <= 8 with a caller-provided FP32
+// factor buffer (LAPACK dsposv's SWORK).
+int dispatch_dsposv_ws(hipStream_t s, int bf16, int itmax, double tol, Mat<double> A, Mat<double> B, Mat<double> X,
+                       float* work, int64_t batch, int32_t* info, int32_t* iter) {
+    if (A.rows <= 128 && X.cols <= 4) return dispatch_dsposv(s, bf16, itmax, tol, A, B, X, batch, info, iter);
+    if (wg_covers(A.rows, X.cols) && work) {
+        int const r = launch_wg_mixed(s, 0, bf16, itmax, tol, A, nullptr, 0, B, X, X.cols, work, batch, info, iter);
+        if (r) return r;
+        return launch_wg_posv(s, A, A, B, X, X.cols, batch, info, iter);
+    }
+    int const r = launch_fill_i32(s, iter, -3, batch);
+    if (r) return r;
+    return dispatch_posv<double>(s, A, A, B, X, batch, info);
+}
+
+// LU with partial pivoting in place of A and the solve (LAPACK dgesv).
+int dispatch_dgesv(hipStream_t s, Mat<double> A, int64_t* ipiv, int64_t ipiv_stride, Mat<double> B, Mat<double> X,
+                   int64_t batch, int32_t* info) {
+    if (algorithm() == BLAST_HIP_ALGO_AUTO && wg_covers(A.rows, X.cols))
+        return launch_wg_gesv(s, A, ipiv, ipiv_stride, B, X, X.cols, batch, info);
+    int const r = launch_getrf_ref<double>(s, A, ipiv, ipiv_stride, batch, info);
+    if (r) return r;
+    return getrs<double>(s, A, ipiv, ipiv_stride, B, X, batch);
+}
+
+// Mixed-precision LU + FP64 refinement (LAPACK dsgesv): instances that do
+// not converge are solved by the FP64 LU in place of A.
+int dispatch_dsgesv(hipStream_t s, int bf16, int itmax, double tol, Mat<double> A, int64_t* ipiv, int64_t ipiv_stride,
+                    Mat<double> B, Mat<double> X, float* work, int64_t batch, int32_t* info, int32_t* iter) {
+    if (wg_covers(A.rows, X.cols) && work) {
+        int const r =
+            launch_wg_mixed(s, 1, bf16, itmax, tol, A, ipiv, ipiv_stride, B, X, X.cols, work, batch, info, iter);
+        if (r) return r;
+        return launch_wg_gesv(s, A, ipiv, ipiv_stride, B, X, X.cols, batch, info, iter);
+    }
+    int const r = launch_fill_i32(s, iter, -3, batch);
+    if (r) return r;
+    return dispatch_dgesv(s, A, ipiv, ipiv_stride, B, X, batch, info);
 }
 
 #define INSTANTIATE(T)                                                                              \

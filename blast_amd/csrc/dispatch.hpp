@@ -50,6 +50,23 @@ int launch_fill_i32(hipStream_t, int32_t* p, int32_t v, int64_t count);
 int dispatch_dsposv(hipStream_t, int bf16, int itmax, double tol, Mat<double> A, Mat<double> B, Mat<double> X,
                     int64_t batch, int32_t* info, int32_t* iter);
 
+// wg_solve.hip: one workgroup per instance, square n <= 256, nrhs <= 8.
+bool wg_covers(int n, int nrhs);
+int64_t wg_work_elems(int n);  // floats per instance of the mixed-precision factor
+int launch_wg_posv(hipStream_t, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int nrhs,
+                   int64_t batch, int32_t* info, const int32_t* only_if = nullptr);
+int launch_wg_gesv(hipStream_t, Mat<double> A, int64_t* ipiv, int64_t ipiv_stride, Mat<double> B, Mat<double> X,
+                   int nrhs, int64_t batch, int32_t* info, const int32_t* only_if = nullptr);
+int launch_wg_mixed(hipStream_t, int lu, int bf16, int itmax, double tol, Mat<double> A, int64_t* ipiv,
+                    int64_t ipiv_stride, Mat<double> B, Mat<double> X, int nrhs, float* work, int64_t batch,
+                    int32_t* info, int32_t* iter);
+int dispatch_dsposv_ws(hipStream_t, int bf16, int itmax, double tol, Mat<double> A, Mat<double> B, Mat<double> X,
+                       float* work, int64_t batch, int32_t* info, int32_t* iter);
+int dispatch_dgesv(hipStream_t, Mat<double> A, int64_t* ipiv, int64_t ipiv_stride, Mat<double> B, Mat<double> X,
+                   int64_t batch, int32_t* info);
+int dispatch_dsgesv(hipStream_t, int bf16, int itmax, double tol, Mat<double> A, int64_t* ipiv, int64_t ipiv_stride,
+                    Mat<double> B, Mat<double> X, float* work, int64_t batch, int32_t* info, int32_t* iter);
+
 // riccati.hip: trailing-update strip width (MFMA tiles per wavefront) per
 // nx bucket (nx / 16), passed by value.
 struct RiccatiTuning {

@@ -20,6 +20,7 @@
 
 #include "common.hpp"
 #include "launch.hpp"
+#include "tiles.hpp"
 
 namespace blast_hip {
 
@@ -184,25 +185,12 @@ __global__ __launch_bounds__(kThreads) void potrf_ref_kernel(Mat<T> A, Mat<T> L,
 }
 
 // ---------------------------------------------------------------- getrf
-// First index of max |x| with BLAST's strict '>' scan (Iamax.hpp:82-95):
-// (value, index) pairs combine as "the later index wins only if strictly
-// greater"; index 0 starts the scan unconditionally, so a NaN there sticks.
-template <typename T>
-__device__ void argmax_combine(T& v, int& i, T v2, int i2) {
-    bool const swap = i2 < i;
-    T const va = swap ? v2 : v, vb = swap ? v : v2;
-    int const ia = swap ? i2 : i, ib = swap ? i : i2;
-    if (vb > va) { v = vb; i = ib; } else { v = va; i = ia; }
-}
-
 template <typename T>
 __device__ int block_iamax(Ref<T> const& a, int r0, int c, int n, T* sv, int* si) {
     T v = T(-1);
     int idx = 0x7fffffff;
     for (int r = threadIdx.x; r < n; r += kThreads) {
-        T const x = fabs(a(r0 + r, c));
-        if (r == 0) { v = x; idx = 0; }
-        else argmax_combine(v, idx, x, r);
+        argmax_combine(v, idx, iamax_key(a(r0 + r, c), r), r);
     }
     for (int o = 32; o > 0; o >>= 1) {
         T const v2 = __shfl_xor(v, o);

@@ -25,20 +25,6 @@
 
 namespace blast_hip {
 
-typedef short s4 __attribute__((ext_vector_type(4)));
-
-// Four consecutive-k operand elements T[l % 16][4 (l / 16) + e], e = 0..3,
-// rounded to BF16 (round to nearest even), for v_mfma_f32_16x16x16bf16_1k.
-__device__ __forceinline__ s4 op_read_bf16(const float* tile, int lane) {
-    s4 v;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        unsigned const u = __builtin_bit_cast(unsigned, tile[(4 * (lane >> 4) + e) * kTileStride + (lane & 15)]);
-        v[e] = short((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
-    }
-    return v;
-}
-
 template <bool PANEL>
 __device__ __forceinline__ double a_sym(Ref<double> const& a, int row, int col) {
     return row >= col ? at<PANEL>(a, row, col) : at<PANEL>(a, col, row);

@@ -52,6 +52,40 @@ template <> struct Mfma<float> {
 
 __device__ __forceinline__ int acc_col(int lane) { return lane & 15; }
 
+// First index of max |x_r| with BLAST's strict '>' scan (dense/Iamax.hpp:82-95),
+// as a reduction over (value, index) pairs.  The scan starts from element 0
+// unconditionally (a NaN there is never replaced) and skips any later NaN
+// (NaN > v is false), so the key of element r is |x_r|, except that a NaN
+// after element 0 becomes -1, below every real |x|.  Pairs combine as "the
+// higher index wins only if strictly greater"; the identity is (-1, INT_MAX).
+template <typename T>
+__device__ __forceinline__ T iamax_key(T x, int r) {
+    T const v = x < T(0) ? -x : x;
+    return (r != 0 && v != v) ? T(-1) : v;
+}
+template <typename T>
+__device__ __forceinline__ void argmax_combine(T& v, int& i, T v2, int i2) {
+    bool const swap = i2 < i;
+    T const va = swap ? v2 : v, vb = swap ? v : v2;
+    int const ia = swap ? i2 : i, ib = swap ? i : i2;
+    if (vb > va) { v = vb; i = ib; } else { v = va; i = ia; }
+}
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+
+// Four consecutive-k operand elements T[l % 16][4 (l / 16) + e], e = 0..3,
+// of an FP32 LDS tile rounded to BF16 (round to nearest even), the operand
+// layout of v_mfma_f32_16x16x16bf16_1k (A and B alike).
+__device__ __forceinline__ s4 op_read_bf16(const float* tile, int lane) {
+    s4 v;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        unsigned const u = __builtin_bit_cast(unsigned, tile[(4 * (lane >> 4) + e) * kTileStride + (lane & 15)]);
+        v[e] = short((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+    }
+    return v;
+}
+
 // Operand element for K-chunk q: T[l % 16][4q + l / 16] of an LDS tile.
 template <typename T>
 __device__ __forceinline__ T op_read(const T* tile, int q, int lane) {

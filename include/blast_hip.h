@@ -210,6 +210,43 @@ int blast_hip_sgetrs_backward_batched(void* stream, const blast_matrix_t* LU, co
                                       int64_t ipiv_stride, const blast_matrix_t* X, const blast_matrix_t* Xbar,
                                       const blast_matrix_t* Abar, const blast_matrix_t* Bbar, int64_t batch);
 
+/* ---- Solves for 128 < n <= 256 (and any n <= 256), nrhs <= 8 ----
+ * BLAST has no solve driver; these compose its getrf (dense/Getrf.hpp:52-119)
+ * or potrf (dense/Potrf.hpp:66-106) with trsm (dense/Trsm.hpp:17-66) and
+ * follow the LAPACK driver contracts named below.  The AUTO algorithm also
+ * routes blast_hip_dpotrf/dposv/dgetrf_batched with square n <= 256 to the
+ * same kernels. */
+
+/* FP32 factor buffer needed per instance by the two mixed-precision calls
+ * below (floats; the buffer holds batch * this many, 0 if n is not covered).
+ * LAPACK's SWORK. */
+int64_t blast_hip_mixed_work_elems(int64_t n);
+
+/* As blast_hip_dsposv_batched, for n <= 256 and nrhs <= 8, with the FP32
+ * factor in `work` (device, batch * blast_hip_mixed_work_elems(n) floats;
+ * unused, may be NULL, when n <= 128 and nrhs <= 4).  LAPACK dsposv. */
+int blast_hip_dsposv_ws_batched(void* stream, int precision, int itmax, double tol, const blast_matrix_t* A,
+                                const blast_matrix_t* B, const blast_matrix_t* X, float* work, int64_t batch,
+                                int32_t* info, int32_t* iter);
+
+/* LU with partial pivoting in place of A (column-major storage), pivots to
+ * ipiv as blast_hip_dgetrf_batched, then X = A^{-1} B.  LAPACK dgesv; info
+ * as getrf (the solve still runs; a zero pivot gives inf/NaN in X). */
+int blast_hip_dgesv_batched(void* stream, const blast_matrix_t* A, int64_t* ipiv, int64_t ipiv_stride,
+                            const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch, int32_t* info);
+
+/* Mixed precision LU (precision 0: FP32; 1: BF16 products with FP32
+ * accumulation in the trailing updates) with FP64 iterative refinement, the
+ * LAPACK dsgesv contract: A is read and left unchanged unless the instance
+ * falls back, in which case A holds its FP64 LU; ipiv holds the pivots of
+ * the factorisation used; iter as dsposv (-1: the low-precision factor hit an
+ * exact zero pivot; -2: no convergence in itmax refinements; -3: shape not
+ * covered, FP64 path used).  work: device, batch *
+ * blast_hip_mixed_work_elems(n) floats. */
+int blast_hip_dsgesv_batched(void* stream, int precision, int itmax, double tol, const blast_matrix_t* A,
+                             int64_t* ipiv, int64_t ipiv_stride, const blast_matrix_t* B, const blast_matrix_t* X,
+                             float* work, int64_t batch, int32_t* info, int32_t* iter);
+
 /* ---- Uneven batched Riccati recursion (LQR / MPC backward pass), FP64 ----
  * No BLAST function is replaced: this composite strings together the entry
  * points above (gemm -> syrkLower -> potrf -> trsm per step) the way the

@@ -54,3 +54,26 @@ def lu_backward_error(a, lu, ipiv):
         if p != i:
             pa[[i, p]] = pa[[p, i]]
     return fro(pa - l @ u) / fro(a)
+
+
+def haar(rng, batch, n):
+    """Haar-distributed orthogonal matrices: QR of a Gaussian matrix with the
+    signs of R's diagonal folded into Q (Mezzadri 2007)."""
+    q, r = np.linalg.qr(rng.standard_normal((batch, n, n)))
+    return q * np.sign(np.diagonal(r, axis1=-2, axis2=-1))[:, None, :]
+
+
+def conditioned(rng, batch, n, kappa, spd=True):
+    """A = U diag(s) V^T (V = U when spd) with s log-uniform in [1/kappa, 1]
+    and both ends present, so cond_2(A) = kappa exactly (kappa: scalar or one
+    value per instance)."""
+    kappa = np.broadcast_to(np.asarray(kappa, dtype=np.float64), (batch,))
+    u = haar(rng, batch, n)
+    v = u if spd else haar(rng, batch, n)
+    t = rng.uniform(0.0, 1.0, (batch, n))
+    t[:, 0], t[:, -1] = 0.0, 1.0
+    s = np.exp(-t * np.log(kappa)[:, None])
+    a = (u * s[:, None, :]) @ np.swapaxes(v, -1, -2)
+    if spd:
+        a = 0.5 * (a + np.swapaxes(a, -1, -2))
+    return a
