@@ -46,7 +46,7 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--riccati-count", type=int, default=20000, help="instances of the uneven Riccati workload")
-    p.add_argument("--only", choices=("all", "posv", "gemm", "riccati"), default="all",
+    p.add_argument("--only", choices=("all", "posv", "gemm", "riccati", "mixed256"), default="all",
                    help="run one workload alone (diagnostics; the bench line needs 'all')")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="first run a 256 MiB device copy (known HBM bytes) to scale rocprofv3 FETCH/WRITE_SIZE")
@@ -250,6 +250,131 @@ def riccati_workload(steps, warmup, count=20000, horizon=10):
     }
 
 
+def conditioned_inputs(batch, n, nrhs, spd, seed):
+    """A = U diag(s) V^T (V = U for the SPD variant), U, V Haar (QR of a
+    Gaussian matrix, signs of R folded in), s log-uniform in [1/kappa, 1] with
+    both ends present, kappa log-uniform in [1e2, 1e8] per instance; B ~ U(-1,1).
+    Generated on the GPU; returns dense column-major A (batch, n, n), B and kappa."""
+    import torch
+
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    kappa = 10.0 ** (2.0 + 6.0 * torch.rand(batch, dtype=torch.float64, device="cuda", generator=gen))
+    a = torch.empty(batch, n, n, dtype=torch.float64, device="cuda")
+
+    def haar(c):
+        q, r = torch.linalg.qr(torch.randn(c, n, n, dtype=torch.float64, device="cuda", generator=gen))
+        return q * torch.sign(torch.diagonal(r, dim1=-2, dim2=-1))[:, None, :]
+
+    for s0 in range(0, batch, 256):
+        c = min(256, batch - s0)
+        u = haar(c)
+        v = u if spd else haar(c)
+        t = torch.rand(c, n, dtype=torch.float64, device="cuda", generator=gen)
+        t[:, 0], t[:, -1] = 0.0, 1.0
+        sv = torch.exp(-t * torch.log(kappa[s0:s0 + c])[:, None])
+        m = torch.bmm(u * sv[:, None, :], v.transpose(1, 2).contiguous())
+        a[s0:s0 + c] = 0.5 * (m + m.transpose(1, 2)) if spd else m
+    b = torch.rand(batch, n, nrhs, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
+    return a, b, kappa
+
+
+def mixed256_workload(steps, warmup, batch=2048, n=256, nrhs=8):
+    """Mixed precision factor + FP64 refinement at n = 256, 8 RHS, cond 1e2..1e8:
+    SPD (Cholesky: FP64 dposv vs dsposv FP32 / BF16) and general (LU with
+    partial pivoting: FP64 dgesv vs dsgesv FP32 / BF16).  Each timed call
+    starts from the pristine A (restored outside the timed region: the FP64
+    LU and every fallback write their factor into A)."""
+    import torch
+
+    from blast_amd import capi
+    from blast_amd.layout import Layout
+    from oracle import oracle
+
+    out = {"batch": batch, "n": n, "nrhs": nrhs, "kappa": "log-uniform [1e2, 1e8] per instance",
+           "stopping_tol": 1e-13}
+    lb = Layout.make(n, nrhs)
+    ib = torch.from_numpy(lb.index_map().reshape(-1)).cuda()
+    work = capi.mixed_work(n, batch)
+    for variant, spd in (("spd", True), ("general", False)):
+        a, b, kappa = conditioned_inputs(batch, n, nrhs, spd, seed=20240611 + spd)
+        la = Layout.make(n, n, panel=spd)
+        ia = torch.from_numpy(la.index_map().reshape(-1)).cuda()
+        a0 = torch.zeros(batch * la.batch_stride, dtype=torch.float64, device="cuda")
+        a0.view(batch, la.batch_stride).index_copy_(1, ia, a.reshape(batch, -1))
+        bbuf = torch.zeros(batch * lb.batch_stride, dtype=torch.float64, device="cuda")
+        bbuf.view(batch, lb.batch_stride).index_copy_(1, ib, b.reshape(batch, -1))
+        abuf, lbuf = torch.empty_like(a0), torch.empty_like(a0)
+        xbuf = torch.zeros_like(bbuf)
+        A, L, B, X = (capi.Operand(la, abuf), capi.Operand(la, lbuf), capi.Operand(lb, bbuf),
+                      capi.Operand(lb, xbuf))
+        it = torch.zeros(batch, dtype=torch.int32, device="cuda")
+        info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+        ipiv = torch.zeros(batch * n, dtype=torch.int64, device="cuda")
+        kdec = torch.floor(torch.log10(kappa)).cpu().numpy().astype(int)
+        res = {}
+        for name, prec in (("fp64", None), ("fp32_ir", 0), ("bf16_ir", 1)):
+            def call():
+                if prec is None:
+                    if spd:
+                        capi.posv(A, L, B, X, batch, info)
+                    else:
+                        capi.gesv(A, ipiv, n, B, X, batch, info)
+                elif spd:
+                    capi.sposv_ws(prec, A, B, X, work, batch, it, info, tol=1e-13)
+                else:
+                    capi.sgesv(prec, A, ipiv, n, B, X, work, batch, it, info, tol=1e-13)
+
+            total = 0.0
+            for k in range(warmup + steps):
+                abuf.copy_(a0)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                call()
+                e1.record()
+                torch.cuda.synchronize()
+                if k >= warmup:
+                    total += e0.elapsed_time(e1) / 1e3
+            t = total / steps
+            x = xbuf.view(batch, lb.batch_stride)[:, ib].reshape(batch, n, nrhs)
+            r = b - torch.bmm(a, x)
+            bwd = (torch.linalg.matrix_norm(r) /
+                   (torch.linalg.matrix_norm(a) * torch.linalg.matrix_norm(x) + torch.linalg.matrix_norm(b)))
+            entry = {"solves_per_s": batch / t, "ms_per_step": t * 1e3,
+                     "max_solve_backward_error": float(bwd.max()), "info_nonzero": int((info != 0).sum())}
+            if prec is not None:
+                its = it.cpu().numpy()
+                entry["refinement_steps"] = {str(v): int((its == v).sum()) for v in sorted(set(its.tolist()))}
+                entry["fell_back_to_fp64"] = int((its < 0).sum())
+                entry["by_kappa_decade"] = {
+                    f"1e{d}": {"instances": int((kdec == d).sum()),
+                               "fallback": int(((kdec == d) & (its < 0)).sum()),
+                               "mean_steps_converged": (float(its[(kdec == d) & (its >= 0)].mean())
+                                                        if ((kdec == d) & (its >= 0)).any() else None)}
+                    for d in range(2, 9) if (kdec == d).any()}
+                entry["speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / (t * 1e3)
+            else:
+                flops = n ** 3 / 3 if spd else 2 * n ** 3 / 3
+                flops += 2 * n * n * nrhs
+                entry["fp64_tflops"] = flops * batch / t / 1e12
+                entry["fp64_frac_of_peak"] = entry["fp64_tflops"] / PEAK_FP64_TFLOPS
+            if prec is None and not spd:
+                # pivots of the FP64 LU vs the oracle on the best-conditioned instances
+                order = torch.argsort(kappa).cpu().numpy()[:8]
+                lo = Layout.make(n, n, panel=False)
+                host = a[order].transpose(1, 2).contiguous().cpu().numpy().reshape(len(order), -1)
+                hbuf = np.zeros(len(order) * lo.batch_stride)
+                hbuf.reshape(len(order), lo.batch_stride)[:, :n * n] = host
+                piv_ref, _ = oracle.getrf(lo, hbuf, len(order))
+                piv = ipiv.view(batch, n)[torch.from_numpy(order).cuda()].cpu().numpy()
+                entry["pivots_equal_oracle"] = f"{int((piv == piv_ref).all(axis=1).sum())}/{len(order)} " \
+                                               "best-conditioned instances"
+            res[name] = entry
+        out[variant] = res
+        del a, b, a0, abuf, lbuf, bbuf, xbuf
+    return out
+
+
 def cpu_baseline(la, lb, abuf, bbuf, target_s):
     from oracle import oracle
 
@@ -288,9 +413,10 @@ def main():
 
     from blast_amd import capi
 
-    if args.only in ("gemm", "riccati"):
-        res = (gemm_workload(args.steps, args.warmup) if args.only == "gemm"
-               else riccati_workload(args.steps, args.warmup, args.riccati_count))
+    if args.only in ("gemm", "riccati", "mixed256"):
+        res = {"gemm": lambda: gemm_workload(args.steps, args.warmup),
+               "riccati": lambda: riccati_workload(args.steps, args.warmup, args.riccati_count),
+               "mixed256": lambda: mixed256_workload(args.steps, args.warmup)}[args.only]()
         print(json.dumps({args.only: res}))
         return
     batch = args.batch
@@ -395,6 +521,7 @@ def main():
     line["config"]["other_workloads"] = {
         "dgemm_64_batch4096": gemm_workload(args.steps, args.warmup),
         f"riccati_uneven_{args.riccati_count}_h10": riccati_workload(args.steps, args.warmup, args.riccati_count),
+        "mixed_n256_nrhs8_batch2048": mixed256_workload(min(args.steps, 5), min(args.warmup, 1)),
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(la, lb, abuf, bbuf, args.cpu_seconds)

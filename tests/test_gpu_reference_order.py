@@ -190,6 +190,28 @@ def test_getrf_rowmajor_and_singular():
     np.testing.assert_array_equal(info.cpu().numpy(), [0, 2, 0])
 
 
+@pytest.mark.parametrize("n", [20, 150])
+def test_getrf_pivot_skips_nan_after_first(n):
+    # BLAST's iamax scans from element 0 with a strict '>' (dense/Iamax.hpp:82-95):
+    # a NaN after the first element is never the pivot.  Element 1 is a NaN
+    # and element 2 the maximum, so the first pivot is row 2, as in the oracle.
+    import torch
+
+    rng = np.random.default_rng(n)
+    a = rng.uniform(-1, 1, (1, n, n))
+    a[0, 1, 0], a[0, 2, 0] = np.nan, 10.0
+    la = Layout.make(n, n, np.float64, panel=False)
+    ah = la.pack(a)
+    piv_ref, _ = oracle.getrf(la, ah.copy(), 1)
+    assert piv_ref[0, 0] == 2
+    for algo in (capi.ALGO_REFERENCE, capi.ALGO_AUTO):
+        capi.set_algorithm(algo)
+        ipiv = torch.full((n,), -1, dtype=torch.int64, device="cuda")
+        capi.getrf(dev(la, ah), ipiv, n, 1)
+        assert int(ipiv[0]) == 2, algo
+    capi.set_algorithm(capi.ALGO_REFERENCE)
+
+
 @pytest.mark.parametrize("n,r", [(1, 1), (17, 3), (128, 4)])
 def test_posv_bitexact(n, r):
     rng = np.random.default_rng(n)
