@@ -518,6 +518,9 @@ def main():
             "fp64_frac_of_peak": FLOPS_PER_INSTANCE * batch / t_plain / 1e12 / PEAK_FP64_TFLOPS,
         },
     }
+    if args.only == "posv":
+        print(json.dumps(line))
+        return
     line["config"]["other_workloads"] = {
         "dgemm_64_batch4096": gemm_workload(args.steps, args.warmup),
         f"riccati_uneven_{args.riccati_count}_h10": riccati_workload(args.steps, args.warmup, args.riccati_count),

@@ -1,0 +1,256 @@
+// Fused batched Cholesky factor + solve, FP64, n <= 256, nrhs <= 8.
+//
+// One workgroup of NW wavefronts per instance; the NT(NT+1)/2 lower 16x16
+// tiles are distributed round-robin over the wavefronts and stay in their
+// owner's registers as v_mfma_f64_16x16x4f64 accumulators for the whole
+// factorisation ("owner computes").  Right-looking blocked Cholesky, block
+// column kb = 0..NT-1, two barriers per block:
+//   1. the owner of tile (kb,kb) factors it (diag_factor: left-looking inside
+//      the tile, Newton-refined rsqrt, DPP row broadcasts), forms
+//      W = L_kk^{-1}, writes L_kk, keeps W in the tile's registers for the
+//      backward solve and forms y_kb = W r_kb;
+//   2. the owners of panel tiles (i,kb) form L_ik = A_ik W^T on MFMA, write
+//      it to L and to LDS, and update r_i -= L_ik y_kb;
+//   3. every owner of a trailing tile (i,j), kb < j <= i, applies
+//      A_ij -= L_ik L_jk^T on MFMA with both operands read from LDS.
+// The backward solve x = L^{-T} y reads W_kk and L_kj straight from the
+// registers: the f64 accumulator layout of a tile M is the MFMA A-operand
+// layout of M^T.  HBM traffic is the algorithmic one: A's lower triangle and
+// B in, L's lower triangle and X out.
+// Algorithm reference: the potrf -> trsm<Lower> -> trsm<Upper>(trans(L))
+// sequence of dense/Potrf.hpp:66-106 and dense/Trsm.hpp:17-66; the tile
+// algorithm reorders BLAST's sums (agreement to backward error, not bits).
+
+#include "dispatch.hpp"
+#include "launch.hpp"
+#include "tiles.hpp"
+
+namespace blast_hip {
+namespace {
+
+constexpr int kOwnRhs = 8;  // right-hand-side columns held in LDS
+
+// Returns v unchanged, opaque to the optimiser: keeps per-tile address
+// arithmetic inside the block loop instead of hoisted (and held in
+// registers) for every tile of the wavefront.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+s"(v));
+    return v;
+}
+
+template <int NT, int NW, int WPE, bool PANEL>
+__global__ __launch_bounds__(64 * NW, WPE) void chol_owner_kernel(Mat<double> A, Mat<double> L, Mat<double> B,
+                                                             Mat<double> X, int n, int nrhs, int32_t* info,
+                                                             const int32_t* only_if) {
+    using Mf = Mfma<double>;
+    constexpr int NTILES = NT * (NT + 1) / 2;
+    constexpr int S = (NTILES + NW - 1) / NW;  // register tiles per wavefront
+    __shared__ double sP[NT][kTileElems];      // panel tiles L_ik of the current block, T[m][k]
+    __shared__ double sD[kTileElems];          // diagonal tile being factored
+    __shared__ double sW[kTileElems];          // W = L_kk^{-1} of the current block
+    __shared__ double sR[NT * 16 * kOwnRhs];   // right-hand sides -> y -> x, [row][kOwnRhs]
+    __shared__ d4 sPark[S][64];                // the diagonal owner's other tiles during diag_factor
+    __shared__ int sBad;
+
+    int64_t const b = blockIdx.x;
+    if (only_if && only_if[b] >= 0) return;
+    int const lane = threadIdx.x & 63;
+    int const w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    Ref<double> a(A, b), l(L, b);
+    bool const lpanel = L.panel != 0;
+
+    // tile t = i(i+1)/2 + j lives in wavefront t % NW, register slot t / NW
+    int ti[S], tj[S];
+    d4 acc[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        int const t = s * NW + w;
+        int i = int((__builtin_sqrtf(8.f * float(t) + 1.f) - 1.f) * 0.5f);
+        while ((i + 1) * (i + 2) / 2 <= t) ++i;
+        while (i * (i + 1) / 2 > t) --i;
+        ti[s] = t < NTILES ? i : -1;
+        tj[s] = t - i * (i + 1) / 2;
+        acc[s] = ti[s] >= 0 ? load_tile<PANEL, double>(a, n, ti[s], tj[s], lane) : d4{0.0, 0.0, 0.0, 0.0};
+    }
+    if (threadIdx.x == 0) sBad = 0;
+    int const nr = nrhs;
+    if (nr > 0) {
+        Ref<double> bb(B, b);
+        for (int e = threadIdx.x; e < NT * 16 * kOwnRhs; e += 64 * NW) {
+            int const row = e / kOwnRhs, c = e % kOwnRhs;
+            sR[e] = (row < n && c < nr) ? bb(row, c) : 0.0;
+        }
+    }
+    __syncthreads();
+
+    // right-hand-side block blk: MFMA B operand row k, column lane % 16
+    auto rhs_b = [&](int blk, int k) -> double {
+        int const c = lane & 15;
+        return c < kOwnRhs ? sR[(16 * blk + k) * kOwnRhs + c] : 0.0;
+    };
+    auto rhs_load = [&](int blk) {
+        d4 v;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int const c = acc_col(lane);
+            v[r] = c < kOwnRhs ? sR[(16 * blk + Mf::row(lane, r)) * kOwnRhs + c] : 0.0;
+        }
+        return v;
+    };
+    auto rhs_store = [&](int blk, d4 v) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int const c = acc_col(lane);
+            if (c < kOwnRhs) sR[(16 * blk + Mf::row(lane, r)) * kOwnRhs + c] = v[r];
+        }
+    };
+
+    for (int kb = 0; kb < NT; ++kb) {
+        {  // 1. diagonal tile, by its owner (one copy of the factor code)
+            int const td = kb * (kb + 1) / 2 + kb, sd = td / NW;
+            if (w == td % NW) {
+                // park the other tiles in LDS: diag_factor needs the registers
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (s == sd)
+                        acc_to_lds(sD, acc[s], lane);
+                    else
+                        sPark[s][lane] = acc[s];
+                }
+                int const bk = diag_factor<double>(sD, sW, lane);
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                    if (s != sd) acc[s] = sPark[s][lane];
+                if (lane == 0 && bk && sBad == 0 && 16 * kb + bk <= n) sBad = 16 * kb + bk;
+                store_tile<PANEL>(l, lpanel, n, kb, kb, lds_to_acc<double>(sD, lane), lane);
+                d4 const wv = lds_to_acc<double>(sW, lane);
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                    if (s == sd) acc[s] = wv;  // W_kk, kept for the backward solve
+                if (nr > 0) {
+                    d4 y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) y = Mf::mma(op_read(sW, q, lane), rhs_b(kb, 4 * q + (lane >> 4)), y);
+                    rhs_store(kb, y);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < S; ++s) {  // 2. panel
+            if (tj[s] == kb && ti[s] > kb) {
+                int const i = opaque(ti[s]);
+                acc_to_lds(sP[i], acc[s], lane);
+                d4 z = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) z = Mf::mma(op_read(sP[i], q, lane), op_read(sW, q, lane), z);
+                acc[s] = z;
+                store_tile<PANEL>(l, lpanel, n, i, opaque(kb), z, lane);
+                acc_to_lds(sP[i], z, lane);
+                if (nr > 0) {
+                    d4 r = rhs_load(i);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        r = Mf::mma(-op_read(sP[i], q, lane), rhs_b(kb, 4 * q + (lane >> 4)), r);
+                    rhs_store(i, r);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < S; ++s) {  // 3. trailing update
+            if (tj[s] > kb) {
+                int const i = opaque(ti[s]), j = opaque(tj[s]);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[s] = Mf::mma(-op_read(sP[i], q, lane), op_read(sP[j], q, lane), acc[s]);
+            }
+        }
+    }
+
+    if (nr > 0) {  // x = L^{-T} y: W_kk and L_kj from the registers (A operand of the transpose)
+        for (int bk = NT - 1; bk >= 0; --bk) {
+            __syncthreads();
+            {
+                int const td = bk * (bk + 1) / 2 + bk, sd = td / NW;
+                if (w == td % NW) {
+                    d4 wv = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int s = 0; s < S; ++s)
+                        if (s == sd) wv = acc[s];
+                    d4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) x = Mf::mma(wv[q], rhs_b(bk, Mf::row(lane, q)), x);
+                    rhs_store(bk, x);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if (ti[s] == bk && tj[s] < bk) {
+                    int const j = opaque(tj[s]);
+                    d4 y = rhs_load(j);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) y = Mf::mma(-acc[s][q], rhs_b(bk, Mf::row(lane, q)), y);
+                    rhs_store(j, y);
+                }
+            }
+        }
+        __syncthreads();
+        Ref<double> xx(X, b);
+        for (int e = threadIdx.x; e < n * nr; e += 64 * NW) {
+            int const row = e % n, c = e / n;
+            xx(row, c) = sR[row * kOwnRhs + c];
+        }
+    } else {
+        __syncthreads();
+    }
+    if (info && threadIdx.x == 0) info[b] = sBad;
+}
+
+template <int NT, int NW, int WPE = 1>
+int launch_owner_nt(hipStream_t s, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int n, int nrhs,
+                    int64_t batch, int32_t* info, const int32_t* only_if) {
+    bool const panel = A.panel && !A.trans && L.panel && !L.trans;
+    if (panel)
+        hipLaunchKernelGGL((chol_owner_kernel<NT, NW, WPE, true>), dim3(unsigned(batch)), dim3(64 * NW), 0, s, A, L, B, X,
+                           n, nrhs, info, only_if);
+    else
+        hipLaunchKernelGGL((chol_owner_kernel<NT, NW, WPE, false>), dim3(unsigned(batch)), dim3(64 * NW), 0, s, A, L, B,
+                           X, n, nrhs, info, only_if);
+    return check_launch("chol_owner_kernel");
+}
+
+}  // namespace
+
+// Fused potrf (+ forward/backward solve when nrhs > 0), square n <= 256,
+// nrhs <= 8.  The dispatcher prefers posv_tiles.hip for n <= 128, nrhs <= 4
+// (one wavefront per instance with every tile in its registers is faster
+// there).  L may alias A, X may alias B.  Returns 1 when the shape is not
+// covered.
+int launch_chol_owner(hipStream_t s, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int64_t batch,
+                      int32_t* info, const int32_t* only_if) {
+    int const n = L.rows, nrhs = (B.data && X.data) ? X.cols : 0;
+    if (L.rows != L.cols || n > 256 || nrhs > kOwnRhs) return 1;
+    if (batch == 0) return 0;
+    switch ((n + 15) / 16) {
+        case 0:
+        case 1: return launch_owner_nt<1, 1>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 2: return launch_owner_nt<2, 1>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 3: return launch_owner_nt<3, 2>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 4: return launch_owner_nt<4, 2>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 5: return launch_owner_nt<5, 4>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 6: return launch_owner_nt<6, 4>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 7: return launch_owner_nt<7, 4>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 8: return launch_owner_nt<8, 4, 2>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 9: return launch_owner_nt<9, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 10: return launch_owner_nt<10, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 11: return launch_owner_nt<11, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 12: return launch_owner_nt<12, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 13: return launch_owner_nt<13, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 14: return launch_owner_nt<14, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 15: return launch_owner_nt<15, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        default: return launch_owner_nt<16, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+    }
+}
+
+}  // namespace blast_hip

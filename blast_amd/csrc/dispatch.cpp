@@ -38,7 +38,8 @@ int dispatch_potrf(hipStream_t s, Mat<T> A, Mat<T> L, int64_t batch, int32_t* in
             Mat<double> none{};
             int const r = launch_posv_tiles(s, A, L, none, none, batch, info);
             if (r != 1) return r;
-            if (A.rows == A.cols && wg_covers(A.rows, 0)) return launch_wg_posv(s, A, L, none, none, 0, batch, info);
+            int const r2 = launch_chol_owner(s, A, L, none, none, batch, info);
+            if (r2 != 1) return r2;
         }
     }
     return launch_potrf_ref<T>(s, A, L, batch, info);
@@ -62,7 +63,8 @@ int dispatch_posv(hipStream_t s, Mat<T> A, Mat<T> L, Mat<T> B, Mat<T> X, int64_t
         if (algorithm() == BLAST_HIP_ALGO_AUTO && X.cols > 0) {
             int const r = launch_posv_tiles(s, A, L, B, X, batch, info);
             if (r != 1) return r;
-            if (wg_covers(A.rows, X.cols)) return launch_wg_posv(s, A, L, B, X, X.cols, batch, info);
+            int const r2 = launch_chol_owner(s, A, L, B, X, batch, info);
+            if (r2 != 1) return r2;
         }
     }
     int r = launch_potrf_ref<T>(s, A, L, batch, info);
@@ -93,7 +95,9 @@ int dispatch_dsposv_ws(hipStream_t s, int bf16, int itmax, double tol, Mat<doubl
     if (wg_covers(A.rows, X.cols) && work) {
         int const r = launch_wg_mixed(s, 0, bf16, itmax, tol, A, nullptr, 0, B, X, X.cols, work, batch, info, iter);
         if (r) return r;
-        return launch_wg_posv(s, A, A, B, X, X.cols, batch, info, iter);
+        int const r2 = launch_posv_tiles(s, A, A, B, X, batch, info, iter);
+        if (r2 != 1) return r2;
+        return launch_chol_owner(s, A, A, B, X, batch, info, iter);  // covers every shape wg_covers accepts
     }
     int const r = launch_fill_i32(s, iter, -3, batch);
     if (r) return r;

@@ -43,6 +43,10 @@ int getrs_backward(hipStream_t, Mat<T> LU, const int64_t* ipiv, int64_t ipiv_str
 // non-null, only instances b with only_if[b] < 0 are processed.
 int launch_posv_tiles(hipStream_t, Mat<double>, Mat<double>, Mat<double>, Mat<double>, int64_t, int32_t*,
                       const int32_t* only_if = nullptr);
+// chol_owner.hip: fused FP64 Cholesky (+ solve) for n <= 256, nrhs <= 8;
+// returns 1 when the shape is not covered.  only_if as launch_posv_tiles.
+int launch_chol_owner(hipStream_t, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int64_t batch,
+                      int32_t* info, const int32_t* only_if = nullptr);
 // sposv_ir.hip: returns 1 when the shape is not covered
 int launch_sposv_ir(hipStream_t, int bf16, Mat<double> A, Mat<double> B, Mat<double> X, int itmax, double tol,
                     int64_t batch, int32_t* info, int32_t* iter);
@@ -53,8 +57,6 @@ int dispatch_dsposv(hipStream_t, int bf16, int itmax, double tol, Mat<double> A,
 // wg_solve.hip: one workgroup per instance, square n <= 256, nrhs <= 8.
 bool wg_covers(int n, int nrhs);
 int64_t wg_work_elems(int n);  // floats per instance of the mixed-precision factor
-int launch_wg_posv(hipStream_t, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int nrhs,
-                   int64_t batch, int32_t* info, const int32_t* only_if = nullptr);
 int launch_wg_gesv(hipStream_t, Mat<double> A, int64_t* ipiv, int64_t ipiv_stride, Mat<double> B, Mat<double> X,
                    int nrhs, int64_t batch, int32_t* info, const int32_t* only_if = nullptr);
 int launch_wg_mixed(hipStream_t, int lu, int bf16, int itmax, double tol, Mat<double> A, int64_t* ipiv,

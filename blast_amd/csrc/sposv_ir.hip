@@ -2,13 +2,14 @@
 // n <= 128, nrhs <= 4 (the LAPACK dsposv contract: A is read, X written;
 // A is overwritten by its FP64 factor only for instances that fall back).
 //
-// One wavefront per instance:
+// One wavefront per instance, two per SIMD (the FP32 factor lives in
+// registers, LDS holds one block column and the vectors, about 17 KB):
 //   1. factor A (rounded to FP32) with the right-looking tile algorithm of
 //      posv_tiles.hip on v_mfma_f32_16x16x4f32 (or, with BF16, trailing
 //      updates on v_mfma_f32_16x16x16bf16_1k: BF16 products, FP32
-//      accumulation); the finished FP32 tiles stay in LDS, each diagonal
-//      tile replaced by W = L_kk^{-1};
-//   2. x = 0, r = b; repeat: d = L^{-T} L^{-1} r in FP32, x += d in FP64,
+//      accumulation); the factor tiles stay in the accumulator registers,
+//      each diagonal tile replaced by W = L_kk^{-1};
+//   2. x = 0, r = b; repeat: d = L^{-T} L^{-1} r in FP32 on MFMA, x += d in FP64,
 //      r = b - A x in FP64 with A read again from HBM (L2/MALL resident),
 //      until max|r| <= max|x| * ||A||_inf * eps * sqrt(n) per right-hand side
 //      (LAPACK dsposv's test, BWDMAX = 1; or tol instead of eps*sqrt(n)
@@ -31,14 +32,15 @@ __device__ __forceinline__ double a_sym(Ref<double> const& a, int row, int col) 
 }
 
 template <int NT, bool PANEL, bool BF16>
-__global__ __launch_bounds__(64) void sposv_ir_kernel(Mat<double> A, Mat<double> B, Mat<double> X, int n, int nrhs,
-                                                      int itmax, double tol, int32_t* info, int32_t* iter) {
+__global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<double> B, Mat<double> X, int n,
+                                                         int nrhs, int itmax, double tol, int32_t* info,
+                                                         int32_t* iter) {
     constexpr int NTILES = NT * (NT + 1) / 2;
     constexpr int NV = NT * 16 * kMaxRhs;
-    __shared__ float sL[NTILES][kTileElems];  // FP32 factor; tile (k,k) holds W_k = L_kk^{-1}
-    __shared__ double sX[NV];                 // x, [row][rhs]
-    __shared__ float sV[NV];                  // FP32 solve vector, [row][rhs]
-    __shared__ float sT[16 * kMaxRhs];        // one block of the substitution
+    __shared__ float sP[NT][kTileElems];  // block column kb (operand layout); sP[kb]: the diagonal tile
+    __shared__ float sW[kTileElems];      // W_kb while factoring; one staged tile in the solves
+    __shared__ double sX[NV];             // x, [row][rhs]
+    __shared__ float sV[NV];              // FP32 solve vector, [row][rhs]
 
     int64_t const b = blockIdx.x;
     int const lane = threadIdx.x;
@@ -47,77 +49,121 @@ __global__ __launch_bounds__(64) void sposv_ir_kernel(Mat<double> A, Mat<double>
     using acc_t = M::acc_t;
 
     // ---------------------------------------------------------- factor
+    // acc holds the FP32 factor for the whole kernel: L_ij below the
+    // diagonal and W_k = L_kk^{-1} on it.
+    acc_t acc[NTILES];
     int bad = 0;
-    {
-        acc_t acc[NTILES];
-        static_for<0, NT>([&](auto I) __attribute__((always_inline)) {
-            constexpr int i = decltype(I)::value;
-            static_for<0, i + 1>([&](auto J) __attribute__((always_inline)) {
-                constexpr int j = decltype(J)::value;
-                acc[tri(i, j)] = load_tile<PANEL, float>(a, n, i, j, lane);
-            });
+    static_for<0, NT>([&](auto I) __attribute__((always_inline)) {
+        constexpr int i = decltype(I)::value;
+        static_for<0, i + 1>([&](auto J) __attribute__((always_inline)) {
+            constexpr int j = decltype(J)::value;
+            acc[tri(i, j)] = load_tile<PANEL, float>(a, n, i, j, lane);
         });
-        static_for<0, NT>([&](auto KB) __attribute__((always_inline)) {
-            constexpr int kb = decltype(KB)::value;
-            static_for<kb, NT>([&](auto I) __attribute__((always_inline)) {
-                constexpr int i = decltype(I)::value;
-                acc_to_lds<float>(sL[tri(i, kb)], acc[tri(i, kb)], lane);
-            });
-            float* const W = sL[tri(kb, kb)];
-            int const bk = diag_factor<float>(W, W, lane);  // second write (W) wins
-            if (bk && bad == 0 && 16 * kb + bk <= n) bad = 16 * kb + bk;
+    });
+    static_for<0, NT>([&](auto KB) __attribute__((always_inline)) {
+        constexpr int kb = decltype(KB)::value;
+        static_for<kb, NT>([&](auto I) __attribute__((always_inline)) {
+            constexpr int i = decltype(I)::value;
+            acc_to_lds<float>(sP[i], acc[tri(i, kb)], lane);
+        });
+        int const bk = diag_factor<float>(sP[kb], sW, lane);
+        if (bk && bad == 0 && 16 * kb + bk <= n) bad = 16 * kb + bk;
+        acc[tri(kb, kb)] = lds_to_acc<float>(sW, lane);
+        static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
+            constexpr int i = decltype(I)::value;
+            acc_t z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) z = M::mma(op_read(sP[i], q, lane), op_read(sW, q, lane), z);
+            acc[tri(i, kb)] = z;
+            acc_to_lds<float>(sP[i], z, lane);
+        });
+        if constexpr (BF16) {
+            s4 f[NT];
             static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
                 constexpr int i = decltype(I)::value;
-                acc_t z = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int q = 0; q < 4; ++q) z = M::mma(op_read(sL[tri(i, kb)], q, lane), op_read(W, q, lane), z);
-                acc_to_lds<float>(sL[tri(i, kb)], z, lane);
+                f[i] = op_read_bf16(sP[i], lane);
             });
-            if constexpr (BF16) {
-                s4 f[NT];
-                static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
-                    constexpr int i = decltype(I)::value;
-                    s4 v = op_read_bf16(sL[tri(i, kb)], lane);
-                    f[i] = v;
-                });
-                static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
-                    constexpr int i = decltype(I)::value;
-                    static_for<kb + 1, i + 1>([&](auto J) __attribute__((always_inline)) {
-                        constexpr int j = decltype(J)::value;
-                        s4 neg = f[i];
+            static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
+                constexpr int i = decltype(I)::value;
+                static_for<kb + 1, i + 1>([&](auto J) __attribute__((always_inline)) {
+                    constexpr int j = decltype(J)::value;
+                    s4 neg = f[i];
 #pragma unroll
-                        for (int e = 0; e < 4; ++e) neg[e] = short(neg[e] ^ short(0x8000));
-                        acc[tri(i, j)] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(neg, f[j], acc[tri(i, j)], 0, 0, 0);
-                    });
+                    for (int e = 0; e < 4; ++e) neg[e] = short(neg[e] ^ short(0x8000));
+                    acc[tri(i, j)] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(neg, f[j], acc[tri(i, j)], 0, 0, 0);
                 });
-            } else {
-                float f[NT][4];
-                static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
-                    constexpr int i = decltype(I)::value;
+            });
+        } else {
+            static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
+                constexpr int i = decltype(I)::value;
+                static_for<kb + 1, i + 1>([&](auto J) __attribute__((always_inline)) {
+                    constexpr int j = decltype(J)::value;
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) f[i][q] = op_read(sL[tri(i, kb)], q, lane);
+                    for (int q = 0; q < 4; ++q)
+                        acc[tri(i, j)] = M::mma(-op_read(sP[i], q, lane), op_read(sP[j], q, lane), acc[tri(i, j)]);
                 });
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
-                        constexpr int i = decltype(I)::value;
-                        static_for<kb + 1, i + 1>([&](auto J) __attribute__((always_inline)) {
-                            constexpr int j = decltype(J)::value;
-                            acc[tri(i, j)] = M::mma(-f[i][q], f[j][q], acc[tri(i, j)]);
-                        });
-                    });
-            }
-        });
-    }
+            });
+        }
+    });
     if (bad) {
         if (lane == 0) iter[b] = -1;
         return;
     }
 
+    // v <- L^{-T} L^{-1} v in FP32 on MFMA, the right-hand sides as the B
+    // operand (columns 0..kMaxRhs-1 of the 16).  Forward: y_k = W_k v_k,
+    // v_i -= L_ik y_k (tiles staged through sW into the A-operand layout).
+    // Backward: x_k = W_k^T v_k, v_j -= L_kj^T x_k, straight from the
+    // registers: register r of an accumulator tile M is the A operand of M^T
+    // for K-chunk r with k = M::row(lane, r).
+    int const cn = lane & 15;
+    auto vb = [&](int blk, int k) -> float { return cn < kMaxRhs ? sV[(16 * blk + k) * kMaxRhs + cn] : 0.f; };
+    auto vload = [&](int blk) {
+        acc_t v;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = cn < kMaxRhs ? sV[(16 * blk + M::row(lane, r)) * kMaxRhs + cn] : 0.f;
+        return v;
+    };
+    auto vstore = [&](int blk, acc_t v) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (cn < kMaxRhs) sV[(16 * blk + M::row(lane, r)) * kMaxRhs + cn] = v[r];
+    };
+    auto solve = [&]() {
+        static_for<0, NT>([&](auto KB) __attribute__((always_inline)) {
+            constexpr int kb = decltype(KB)::value;
+            acc_to_lds<float>(sW, acc[tri(kb, kb)], lane);
+            acc_t y = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) y = M::mma(op_read(sW, q, lane), vb(kb, 4 * q + (lane >> 4)), y);
+            vstore(kb, y);
+            static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
+                constexpr int i = decltype(I)::value;
+                acc_to_lds<float>(sW, acc[tri(i, kb)], lane);
+                acc_t v = vload(i);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v = M::mma(-op_read(sW, q, lane), vb(kb, 4 * q + (lane >> 4)), v);
+                vstore(i, v);
+            });
+        });
+        static_for<0, NT>([&](auto KK) __attribute__((always_inline)) {
+            constexpr int kb = NT - 1 - decltype(KK)::value;
+            acc_t x = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x = M::mma(acc[tri(kb, kb)][q], vb(kb, M::row(lane, q)), x);
+            vstore(kb, x);
+            static_for<0, kb>([&](auto J) __attribute__((always_inline)) {
+                constexpr int j = decltype(J)::value;
+                acc_t v = vload(j);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v = M::mma(-acc[tri(kb, j)][q], vb(kb, M::row(lane, q)), v);
+                vstore(j, v);
+            });
+        });
+    };
+
     // ---------------------------------------------------------- refine
-    // Solve lanes: (m, c) = (row within a 16-row block, right-hand side).
     // Residual lanes: one row each, 64 rows per pass.
-    int const m = lane >> 2, c = lane & 3;
     for (int e = lane; e < NV; e += 64) {
         int const row = e >> 2, cc = e & 3;
         sX[e] = 0.0;
@@ -127,39 +173,7 @@ __global__ __launch_bounds__(64) void sposv_ir_kernel(Mat<double> A, Mat<double>
     int it = 0;
     bool converged = false;
     for (;; ++it) {
-        // d = L^{-T} L^{-1} v in FP32, by 16-row blocks
-        static_for<0, NT>([&](auto KB) __attribute__((always_inline)) {
-            constexpr int kb = decltype(KB)::value;
-            float t = sV[(16 * kb + m) * kMaxRhs + c];
-            static_for<0, kb>([&](auto J) __attribute__((always_inline)) {
-                constexpr int j = decltype(J)::value;
-                const float* Lt = sL[tri(kb, j)];
-#pragma unroll
-                for (int k = 0; k < 16; ++k) t = __builtin_fmaf(-Lt[k * kTileStride + m], sV[(16 * j + k) * kMaxRhs + c], t);
-            });
-            sT[m * kMaxRhs + c] = t;
-            const float* W = sL[tri(kb, kb)];
-            float y = 0.f;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) y = __builtin_fmaf(W[k * kTileStride + m], sT[k * kMaxRhs + c], y);
-            sV[(16 * kb + m) * kMaxRhs + c] = y;
-        });
-        static_for<0, NT>([&](auto KK) __attribute__((always_inline)) {
-            constexpr int kb = NT - 1 - decltype(KK)::value;
-            float t = sV[(16 * kb + m) * kMaxRhs + c];
-            static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
-                constexpr int i = decltype(I)::value;
-                const float* Lt = sL[tri(i, kb)];
-#pragma unroll
-                for (int k = 0; k < 16; ++k) t = __builtin_fmaf(-Lt[m * kTileStride + k], sV[(16 * i + k) * kMaxRhs + c], t);
-            });
-            sT[m * kMaxRhs + c] = t;
-            const float* W = sL[tri(kb, kb)];
-            float x = 0.f;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) x = __builtin_fmaf(W[m * kTileStride + k], sT[k * kMaxRhs + c], x);
-            sV[(16 * kb + m) * kMaxRhs + c] = x;
-        });
+        solve();  // d = L^{-T} L^{-1} v
         for (int e = lane; e < NV; e += 64) sX[e] += double(sV[e]);
         // r = b - A x in FP64, A (symmetric, lower triangle stored) read in
         // 16-column batches so the loads overlap; v = float(r)

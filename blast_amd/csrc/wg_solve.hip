@@ -1,7 +1,8 @@
-// Batched factor + solve for n <= 256, nrhs <= 8: Cholesky (potrf + two
-// trsm) or LU with partial pivoting (getrf + getrs), in FP64, or in FP32
+// Batched factor + solve for n <= 256, nrhs <= 8: LU with partial pivoting
+// (getrf + getrs) in FP64, and Cholesky (potrf + two trsm) or LU in FP32
 // (optionally BF16 products in the trailing updates) with FP64 iterative
-// refinement (LAPACK dsposv / dsgesv contracts).
+// refinement (LAPACK dgesv / dsposv / dsgesv contracts).  The FP64 Cholesky
+// has its own register-resident kernels (posv_tiles.hip, chol_owner.hip).
 //
 // One 256-thread workgroup (4 wavefronts) per instance.  The factor F lives
 // in global memory (the caller's L / A buffer in FP64, a dense work buffer in
@@ -36,7 +37,7 @@ namespace {
 
 constexpr int kWgThreads = 256;
 constexpr int kWgWaves = kWgThreads / 64;
-constexpr int kWgMaxNT = 16;  // n <= 256
+constexpr int kWgMaxNT = 16;  // n <= 256 (the solves assume at most 4 tiles per wavefront per step)
 constexpr int kWgRhs = 8;     // right-hand-side columns held per instance
 
 struct WgArgs {
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
     __shared__ unsigned long long sMax[2 * kWgRhs];
     __shared__ T sRedV[kWgWaves];
     __shared__ int sRedI[kWgWaves];
-    __shared__ double sAnrm;
+    __shared__ unsigned long long sAnrmBits;  // ||A||_inf as the bits of a non-negative double
     __shared__ int sFlag;
 
     // factor phase: block column (and LU: block row) tiles, T[m][k] at k*17+m
@@ -97,34 +98,23 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
 
     if (tid == 0) sFlag = 0;
     if (kMixed && tid == 0 && g.info) g.info[b] = 0;  // an FP64 fallback overwrites it
-    // ---- ||A||_inf (refinement test) and F = A (converted; Cholesky: lower triangle)
-    if constexpr (kMixed) {
-        double m = 0.0;
-        for (int r = tid; r < n; r += kWgThreads) {
-            double s = 0.0;
-            for (int c = 0; c < n; ++c) s += __builtin_fabs(asym(r, c));
-            m = (s <= m) ? m : s;  // NaN propagates
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            double const m2 = __shfl_xor(m, o);
-            m = (m2 <= m) ? m : m2;
-        }
-        if (lane == 0) reinterpret_cast<double*>(sBuf)[w] = m;
-        __syncthreads();
-        if (tid == 0) {
-            double mm = 0.0;
-            for (int q = 0; q < kWgWaves; ++q) {
-                double const v = reinterpret_cast<double*>(sBuf)[q];
-                mm = (v <= mm) ? mm : v;
-            }
-            sAnrm = mm;
-        }
-        __syncthreads();
-    }
+    if (tid == 0) sAnrmBits = 0ull;
+    // ---- F = A (converted; Cholesky: lower triangle), eight loads in flight per thread
     if (static_cast<const void*>(F.data) != static_cast<const void*>(g.A.data) || F.stride != g.A.stride) {
-        for (int e = tid; e < n * n; e += kWgThreads) {
-            int const r = e % n, c = e / n;
-            if (LU || r >= c) f(r, c) = T(a(r, c));
+        constexpr int U = 8;
+        int const nn = n * n;
+        for (int e0 = tid; e0 < nn; e0 += kWgThreads * U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                int const e = e0 + u * kWgThreads, r = e % n, c = e / n;
+                v[u] = (e < nn && (LU || r >= c)) ? a(r, c) : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                int const e = e0 + u * kWgThreads, r = e % n, c = e / n;
+                if (e < nn && (LU || r >= c)) f(r, c) = T(v[u]);
+            }
         }
     }
     __syncthreads();
@@ -132,9 +122,21 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
     // ---- factorisation
     for (int kb = 0; kb < NT; ++kb) {
         int const r0 = 16 * kb;
-        for (int e = tid; e < (NT - kb) * 256; e += kWgThreads) {
-            int const t = kb + (e >> 8), c = (e >> 4) & 15, m = e & 15;
-            sP[t][c * kTileStride + m] = (!LU && t == kb && m < c) ? T(0) : fget(16 * t + m, r0 + c);
+        {
+            int const ne = (NT - kb) * 256;
+            for (int e0 = tid; e0 < ne; e0 += 4 * kWgThreads) {
+                T v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    int const e = e0 + u * kWgThreads, t = kb + (e >> 8), c = (e >> 4) & 15, m = e & 15;
+                    v[u] = (e >= ne || (!LU && t == kb && m < c)) ? T(0) : fget(16 * t + m, r0 + c);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    int const e = e0 + u * kWgThreads, t = kb + (e >> 8), c = (e >> 4) & 15, m = e & 15;
+                    if (e < ne) sP[t][c * kTileStride + m] = v[u];
+                }
+            }
         }
         __syncthreads();
         if constexpr (!LU) {
@@ -157,26 +159,43 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
             }
             __syncthreads();
             int const mt = NT - kb - 1, ntr = mt * (mt + 1) / 2;
-            for (int t = w; t < ntr; t += kWgWaves) {  // A_ij -= L_ik L_jk^T, kb < j <= i
-                int ii = int((__builtin_sqrtf(8.f * float(t) + 1.f) - 1.f) * 0.5f);
-                while ((ii + 1) * (ii + 2) / 2 <= t) ++ii;
-                while (ii * (ii + 1) / 2 > t) --ii;
-                int const i = kb + 1 + ii, j = kb + 1 + (t - ii * (ii + 1) / 2);
-                acc_t acc;
+            constexpr int U = 4;  // tiles per wavefront per pass: their loads overlap
+            for (int t0 = w; t0 < ntr; t0 += kWgWaves * U) {  // A_ij -= L_ik L_jk^T, kb < j <= i
+                acc_t acc[U];
+                int ti[U], tj[U];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[r] = fget(16 * i + Mf::row(lane, r), 16 * j + acc_col(lane));
-                if constexpr (BF16) {
-                    acc_t const p = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(op_read_bf16(sP[i], lane),
-                                                                             op_read_bf16(sP[j], lane), acc_t{}, 0, 0, 0);
-                    acc = acc - p;
-                } else {
+                for (int u = 0; u < U; ++u) {
+                    int const t = t0 + u * kWgWaves;
+                    int ii = int((__builtin_sqrtf(8.f * float(t) + 1.f) - 1.f) * 0.5f);
+                    while ((ii + 1) * (ii + 2) / 2 <= t) ++ii;
+                    while (ii * (ii + 1) / 2 > t) --ii;
+                    ti[u] = t < ntr ? kb + 1 + ii : -1;
+                    tj[u] = kb + 1 + (t - ii * (ii + 1) / 2);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) acc = Mf::mma(-op_read(sP[i], q, lane), op_read(sP[j], q, lane), acc);
+                    for (int r = 0; r < 4; ++r)
+                        acc[u][r] = ti[u] >= 0 ? fget(16 * ti[u] + Mf::row(lane, r), 16 * tj[u] + acc_col(lane)) : T(0);
                 }
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    int const row = 16 * i + Mf::row(lane, r), col = 16 * j + acc_col(lane);
-                    if (i != j || row >= col) fset(row, col, acc[r]);
+                for (int u = 0; u < U; ++u) {
+                    if (ti[u] < 0) continue;
+                    if constexpr (BF16) {
+                        acc_t const p = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                            op_read_bf16(sP[ti[u]], lane), op_read_bf16(sP[tj[u]], lane), acc_t{}, 0, 0, 0);
+                        acc[u] = acc[u] - p;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            acc[u] = Mf::mma(-op_read(sP[ti[u]], q, lane), op_read(sP[tj[u]], q, lane), acc[u]);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (ti[u] < 0) continue;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        int const row = 16 * ti[u] + Mf::row(lane, r), col = 16 * tj[u] + acc_col(lane);
+                        if (ti[u] != tj[u] || row >= col) fset(row, col, acc[u][r]);
+                    }
                 }
             }
             __syncthreads();
@@ -226,15 +245,41 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
                 fset(16 * t + m, r0 + c, sP[t][c * kTileStride + m]);
             }
             if (tid < 16 && r0 + tid < n) g.ipiv[b * g.ipiv_stride + r0 + tid] = sPiv[r0 + tid];
-            for (int j = tid; j < n; j += kWgThreads) {  // laswp on the other columns
+            for (int j = tid; j < n; j += kWgThreads) {  // laswp on the other columns, in registers
                 if (j >= r0 && j < r0 + 16) continue;
+                // rows r0..r0+15 and the pivot rows below the block: load all, swap, store
+                T v[16], o[16];
+                int pr[16];
+#pragma unroll
                 for (int c = 0; c < 16; ++c) {
-                    int const p = sPiv[r0 + c];
-                    if (p != r0 + c) {
-                        T const t = f(r0 + c, j);
-                        f(r0 + c, j) = f(p, j);
-                        f(p, j) = t;
+                    pr[c] = sPiv[r0 + c];
+                    v[c] = r0 + c < n ? f(r0 + c, j) : T(0);
+                    o[c] = pr[c] >= r0 + 16 && pr[c] < n ? f(pr[c], j) : T(0);
+                }
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    int const p = pr[c];
+                    if (p == r0 + c) continue;
+                    if (p < r0 + 16) {  // both rows inside the block
+#pragma unroll
+                        for (int d = c + 1; d < 16; ++d)
+                            if (p == r0 + d) {
+                                T const t = v[c];
+                                v[c] = v[d], v[d] = t;
+                            }
+                    } else {  // the current content of row p: the latest earlier swap with it, else memory
+                        T cur = o[c];
+#pragma unroll
+                        for (int d = 0; d < c; ++d)
+                            if (pr[d] == p) cur = o[d];
+                        o[c] = v[c];
+                        v[c] = cur;
                     }
+                }
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    if (r0 + c < n) f(r0 + c, j) = v[c];
+                    if (pr[c] >= r0 + 16 && pr[c] < n) f(pr[c], j) = o[c];  // in c order: the last swap wins
                 }
             }
             if (tid < 32) {  // inverses of the unit-lower and upper diagonal blocks, one column per lane
@@ -279,21 +324,39 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
             }
             __syncthreads();
             int const mt = NT - kb - 1;
-            for (int t = w; t < mt * mt; t += kWgWaves) {  // A_ij -= L_ik U_kj
-                int const i = kb + 1 + t / mt, j = kb + 1 + t % mt;
-                acc_t acc;
+            constexpr int U = 4;
+            for (int t0 = w; t0 < mt * mt; t0 += kWgWaves * U) {  // A_ij -= L_ik U_kj
+                acc_t acc[U];
+                int ti[U], tj[U];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[r] = fget(16 * i + Mf::row(lane, r), 16 * j + acc_col(lane));
-                if constexpr (BF16) {
-                    acc_t const p = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(op_read_bf16(sP[i], lane),
-                                                                             op_read_bf16(sU[j], lane), acc_t{}, 0, 0, 0);
-                    acc = acc - p;
-                } else {
+                for (int u = 0; u < U; ++u) {
+                    int const t = t0 + u * kWgWaves;
+                    ti[u] = t < mt * mt ? kb + 1 + t / mt : -1;
+                    tj[u] = kb + 1 + t % mt;
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) acc = Mf::mma(-op_read(sP[i], q, lane), op_read(sU[j], q, lane), acc);
+                    for (int r = 0; r < 4; ++r)
+                        acc[u][r] = ti[u] >= 0 ? fget(16 * ti[u] + Mf::row(lane, r), 16 * tj[u] + acc_col(lane)) : T(0);
                 }
 #pragma unroll
-                for (int r = 0; r < 4; ++r) fset(16 * i + Mf::row(lane, r), 16 * j + acc_col(lane), acc[r]);
+                for (int u = 0; u < U; ++u) {
+                    if (ti[u] < 0) continue;
+                    if constexpr (BF16) {
+                        acc_t const p = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                            op_read_bf16(sP[ti[u]], lane), op_read_bf16(sU[tj[u]], lane), acc_t{}, 0, 0, 0);
+                        acc[u] = acc[u] - p;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            acc[u] = Mf::mma(-op_read(sP[ti[u]], q, lane), op_read(sU[tj[u]], q, lane), acc[u]);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (ti[u] < 0) continue;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        fset(16 * ti[u] + Mf::row(lane, r), 16 * tj[u] + acc_col(lane), acc[u][r]);
+                }
             }
             __syncthreads();
         }
@@ -366,12 +429,24 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
                 rhs_store(bk, acc);
             }
             __syncthreads();
-            for (int i = bk + 1 + w; i < NT; i += kWgWaves) {
-                acc_t acc = rhs_load(i);
+            {  // r_i -= L_ib y_b, up to four tiles per wavefront, their loads issued together
+                T fv[4][4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    acc = Mf::mma(-fget(16 * i + (lane & 15), 16 * bk + 4 * q + (lane >> 4)), rhs_op(bk, q), acc);
-                rhs_store(i, acc);
+                for (int u = 0; u < 4; ++u) {
+                    int const i = bk + 1 + w + kWgWaves * u;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        fv[u][q] = i < NT ? fget(16 * i + (lane & 15), 16 * bk + 4 * q + (lane >> 4)) : T(0);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    int const i = bk + 1 + w + kWgWaves * u;
+                    if (i >= NT) continue;
+                    acc_t acc = rhs_load(i);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc = Mf::mma(-fv[u][q], rhs_op(bk, q), acc);
+                    rhs_store(i, acc);
+                }
             }
             __syncthreads();
         }
@@ -387,14 +462,26 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
                 rhs_store(bk, acc);
             }
             __syncthreads();
-            for (int i = w; i < bk; i += kWgWaves) {
-                acc_t acc = rhs_load(i);
+            {  // y_i -= U_ib x_b, i < b
+                T fv[4][4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    int const m = 16 * i + (lane & 15), k = 16 * bk + 4 * q + (lane >> 4);
-                    acc = Mf::mma(-(LU ? fget(m, k) : fget(k, m)), rhs_op(bk, q), acc);
+                for (int u = 0; u < 4; ++u) {
+                    int const i = w + kWgWaves * u;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        int const m = 16 * i + (lane & 15), k = 16 * bk + 4 * q + (lane >> 4);
+                        fv[u][q] = i < bk ? (LU ? fget(m, k) : fget(k, m)) : T(0);
+                    }
                 }
-                rhs_store(i, acc);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    int const i = w + kWgWaves * u;
+                    if (i >= bk) continue;
+                    acc_t acc = rhs_load(i);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc = Mf::mma(-fv[u][q], rhs_op(bk, q), acc);
+                    rhs_store(i, acc);
+                }
             }
             __syncthreads();
         }
@@ -403,22 +490,38 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
         for (int e = tid; e < npad * kWgRhs; e += kWgThreads) sX[e] += double(sR[e]);
         __syncthreads();
     };
-    // r = b - A x in FP64 into sR; returns the stopping test
-    double const cte = kMixed ? sAnrm * (g.tol > 0.0 ? g.tol : 1.1102230246251565e-16 * __builtin_sqrt(double(n))) : 0.0;
-    auto residual = [&]() -> bool {
+    // r = b - A x in FP64 into sR; returns the stopping test.  The first call
+    // also forms ||A||_inf from the same loads (row sums of |A|).
+    auto residual = [&](bool first) -> bool {
         if (tid < 2 * kWgRhs) sMax[tid] = 0ull;
         __syncthreads();
         for (int i = w; i < NT; i += kWgWaves) {
             Md::acc_t acc{};
             int const m = 16 * i + (lane & 15);
-            for (int j = 0; j < NT; ++j) {
+            double rs = 0.0;
+            for (int j0 = 0; j0 < NT; j0 += 4) {
+                double av[4][4], xv[4][4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    int const k = 16 * j + 4 * q + (lane >> 4);
-                    double const av = (m < n && k < n) ? asym(m, k) : 0.0;
-                    double const xv = (lane & 15) < kWgRhs ? sX[k * kWgRhs + (lane & 15)] : 0.0;
-                    acc = Md::mma(av, xv, acc);
-                }
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        int const k = 16 * (j0 + u) + 4 * q + (lane >> 4);
+                        bool const in = j0 + u < NT;
+                        av[u][q] = (in && m < n && k < n) ? asym(m, k) : 0.0;
+                        xv[u][q] = (in && (lane & 15) < kWgRhs) ? sX[k * kWgRhs + (lane & 15)] : 0.0;
+                    }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        acc = Md::mma(av[u][q], xv[u][q], acc);
+                        if (first) rs += __builtin_fabs(av[u][q]);
+                    }
+            }
+            if (first) {
+                rs += __shfl_xor(rs, 16);
+                rs += __shfl_xor(rs, 32);
+                if (lane < 16) atomicMax(&sAnrmBits, abs_bits(rs));
             }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -435,6 +538,8 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
             }
         }
         __syncthreads();
+        double const cte = __builtin_bit_cast(double, sAnrmBits) *
+                           (g.tol > 0.0 ? g.tol : 1.1102230246251565e-16 * __builtin_sqrt(double(n)));
         bool ok = true;
         for (int c = 0; c < nrhs; ++c) {
             double const rn = __builtin_bit_cast(double, sMax[c]), xn = __builtin_bit_cast(double, sMax[kWgRhs + c]);
@@ -448,7 +553,7 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
     if constexpr (kMixed) {
         int it = 0, status;
         for (;;) {
-            if (residual()) {
+            if (residual(it == 0)) {
                 status = it;
                 break;
             }
@@ -495,13 +600,6 @@ int64_t wg_work_elems(int n) {
 }
 
 bool wg_covers(int n, int nrhs) { return n >= 1 && n <= 16 * kWgMaxNT && nrhs >= 0 && nrhs <= kWgRhs; }
-
-// FP64 Cholesky (L may alias A) and, when nrhs > 0, the solve into X.
-int launch_wg_posv(hipStream_t s, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int nrhs,
-                   int64_t batch, int32_t* info, const int32_t* only_if) {
-    return launch_wg<double, false, false>(s, make_args(A, B, X, nrhs, 0, 0.0, info, nullptr, only_if, nullptr, 0),
-                                           L, batch);
-}
 
 // FP64 LU with partial pivoting in place of A, pivots to ipiv, solve into X.
 int launch_wg_gesv(hipStream_t s, Mat<double> A, int64_t* ipiv, int64_t ipiv_stride, Mat<double> B, Mat<double> X,

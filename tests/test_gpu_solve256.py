@@ -62,6 +62,29 @@ def test_potrf_and_posv_vs_oracle(n, panel):
     assert np.all(potrf_backward_error(a, A.logical(batch)) < 4 * n * EPS64)
 
 
+@pytest.mark.parametrize("n", [1, 17, 64, 100, 128])
+@pytest.mark.parametrize("nrhs", [5, 8])
+def test_posv_many_rhs_small_n(n, nrhs):
+    # 5..8 right-hand sides take the multi-wavefront register kernel at any n
+    rng = np.random.default_rng(n * 100 + nrhs)
+    batch = 3
+    a = spd(rng, batch, n)
+    b = rng.uniform(-1, 1, (batch, n, nrhs))
+    la, lb = Layout.make(n, n), Layout.make(n, nrhs)
+    ah, bh = la.pack(a), lb.pack(b)
+    lh, xh = np.full_like(ah, 3.0), np.zeros_like(bh)
+    oracle.posv(la, ah, la, lh, lb, bh, lb, xh, batch)
+    L, X = capi.Operand.empty(la, batch, fill=3.0), capi.Operand.empty(lb, batch)
+    info = i32(batch, -1)
+    capi.posv(dev(la, ah), L, dev(lb, bh), X, batch, info)
+    assert np.all(info.cpu().numpy() == 0)
+    assert np.all(solve_backward_error(a, X.logical(batch), b) < 4 * max(n, 8) * EPS64)
+    lg, l_ref = L.logical(batch), la.unpack(lh, batch)
+    low = np.tril(np.ones((n, n), bool))
+    np.testing.assert_allclose(lg[:, low], l_ref[:, low], rtol=0, atol=1e-13 * np.abs(l_ref).max())
+    np.testing.assert_array_equal(lg[:, ~low], l_ref[:, ~low])
+
+
 def test_potrf_not_spd_reports_first_column():
     rng = np.random.default_rng(2)
     n, batch = 200, 3
