@@ -46,7 +46,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--riccati-count", type=int, default=20000, help="instances of the uneven Riccati workload")
-    p.add_argument("--only", choices=("all", "posv", "gemm", "riccati", "mixed256"), default="all",
+    p.add_argument("--only", choices=("all", "posv", "gemm", "riccati", "mixed256", "adjoint", "convert"),
+                   default="all",
                    help="run one workload alone (diagnostics; the bench line needs 'all')")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="first run a 256 MiB device copy (known HBM bytes) to scale rocprofv3 FETCH/WRITE_SIZE")
@@ -158,6 +159,28 @@ def gemm_workload(steps, warmup, batch=4096, n=64):
     return {"gflops": flops / t / 1e9, "instances_per_s": batch / t, "ms_per_step": t * 1e3,
             "hbm_gbs_algorithmic": hbm / t / 1e9, "hbm_frac": hbm / t / 1e9 / PEAK_HBM_GBS,
             "kernel": "gemm_tile_kernel<double, 0>"}
+
+
+def convert_workload(steps, warmup, batch=4096, n=128):
+    """Storage conversion of 4096 FP64 128 x 128 matrices: panel -> dense
+    column-major, and panel -> dense row-major (transposing); HBM-bound,
+    8 * n^2 bytes read + 8 * n^2 written per instance."""
+    import torch
+
+    from blast_amd import capi
+    from blast_amd.layout import Layout
+
+    lp, lc, lr = Layout.make(n, n), Layout.make(n, n, panel=False), Layout.make(n, n, panel=False).t()
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    S = capi.Operand(lp, torch.rand(batch * lp.batch_stride, dtype=torch.float64, device="cuda", generator=gen))
+    out = {}
+    for name, ld in (("panel_to_colmajor", lc), ("panel_to_rowmajor", lr)):
+        D = capi.Operand(ld, torch.zeros(batch * ld.batch_stride, dtype=torch.float64, device="cuda"))
+        t = time_steps(lambda: capi.copy(S, D, batch), steps, warmup)
+        gbs = 2 * 8 * n * n * batch / t / 1e9
+        out[name] = {"ms_per_step": t * 1e3, "hbm_gbs_algorithmic": gbs, "hbm_frac": gbs / PEAK_HBM_GBS}
+    out["kernel"] = "convert_kernel<double>"
+    return out
 
 
 def riccati_inputs(count=20000, horizon=10, seed=20240611):
@@ -375,6 +398,123 @@ def mixed256_workload(steps, warmup, batch=2048, n=256, nrhs=8):
     return out
 
 
+NSTREAMS = 8
+
+
+def adjoint_workload(steps, warmup, batch=4096, nmin=32, nmax=192, seed=20240611):
+    """Adjoint of the batched SPD solve x = A^{-1} b: n ~ U{32..192} per
+    instance, A = G G^T + n I, b and the cotangent xbar ~ N(0,1), one RHS.
+    Instances are grouped by n (one uniform batched call per size); the
+    per-size call sequence is timed eagerly and replayed from a captured
+    hipGraph.  FP64: forward dposv (keeps L) + backward from L, and the
+    recompute mode (backward from A, L recomputed into Abar's buffer); FP32:
+    backward from L.  A sample per size is checked against the FP64
+    reference adjoint (Abar = -sym(bbar x^T) folded to the lower entries)."""
+    import torch
+
+    from blast_amd import capi
+    from blast_amd.layout import Layout
+    from oracle import adjoint
+
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(nmin, nmax + 1, size=batch)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    groups = []
+    for n in np.unique(sizes):
+        n, cnt = int(n), int((sizes == n).sum())
+        g = torch.randn(cnt, n, n, dtype=torch.float64, device="cuda", generator=gen)
+        a = torch.bmm(g, g.transpose(1, 2).contiguous()) + n * torch.eye(n, dtype=torch.float64, device="cuda")
+        bv = torch.randn(cnt, n, 1, dtype=torch.float64, device="cuda", generator=gen)
+        xb = torch.randn(cnt, n, 1, dtype=torch.float64, device="cuda", generator=gen)
+        ent = {"n": n, "count": cnt, "a": a, "b": bv, "xb": xb}
+        for dt, tdt in ((np.float64, torch.float64), (np.float32, torch.float32)):
+            la, lb = Layout.make(n, n, dt), Layout.make(n, 1, dt)
+            ia = torch.from_numpy(la.index_map().reshape(-1)).cuda()
+            ib = torch.from_numpy(lb.index_map().reshape(-1)).cuda()
+
+            def pack(x, lay, idx):
+                buf = torch.zeros(cnt * lay.batch_stride, dtype=tdt, device="cuda")
+                # index_map: storage offset of logical (i, j), in row-major (i, j) order
+                buf.view(cnt, lay.batch_stride).index_copy_(1, idx, x.to(tdt).reshape(cnt, -1))
+                return capi.Operand(lay, buf)
+
+            ops = {"A": pack(a, la, ia), "B": pack(bv, lb, ib), "Xb": pack(xb, lb, ib)}
+            ops["L"] = capi.Operand(la, torch.zeros_like(ops["A"].tensor))
+            ops["X"] = capi.Operand(lb, torch.zeros_like(ops["B"].tensor))
+            ops["Ab"] = capi.Operand(la, torch.zeros_like(ops["A"].tensor))
+            ops["Bb"] = capi.Operand(lb, torch.zeros_like(ops["B"].tensor))
+            ops["ia"], ops["ib"] = ia, ib
+            ent[dt.__name__] = ops
+        groups.append(ent)
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    # the forward solves (FP64 fused posv; FP32 potrf + two trsm) give L and x
+    for ent in groups:
+        o = ent["float64"]
+        capi.posv(o["A"], o["L"], o["B"], o["X"], ent["count"], info)
+        o = ent["float32"]
+        capi.potrf(o["A"], o["L"], ent["count"], info)
+        capi.trsm(0, 0, 0, 1.0, o["L"], o["B"], o["X"], ent["count"])
+        capi.trsm(0, 1, 0, 1.0, o["L"].t(), o["X"], o["X"], ent["count"])
+    torch.cuda.synchronize()
+
+    # the size groups are spread over a few streams (default priority), forked
+    # from and joined back into the current stream, so small groups overlap
+    streams = [torch.cuda.Stream() for _ in range(NSTREAMS)]
+
+    def run(kind):
+        cur = torch.cuda.current_stream()
+        for st in streams:
+            st.wait_stream(cur)
+        for g, ent in enumerate(groups):
+            c, sp = ent["count"], streams[g % NSTREAMS].cuda_stream
+            if kind == "fp64_forward":
+                o = ent["float64"]
+                capi.posv(o["A"], o["L"], o["B"], o["X"], c, stream=sp)
+            elif kind == "fp64_backward":
+                o = ent["float64"]
+                capi.posv_backward(0, o["L"], o["X"], o["Xb"], o["Ab"], o["Bb"], c, stream=sp)
+            elif kind == "fp64_backward_recompute":
+                o = ent["float64"]
+                capi.posv_backward(1, o["A"], o["X"], o["Xb"], o["Ab"], o["Bb"], c, stream=sp)
+            else:
+                o = ent["float32"]
+                capi.posv_backward(0, o["L"], o["X"], o["Xb"], o["Ab"], o["Bb"], c, stream=sp)
+        for st in streams:
+            cur.wait_stream(st)
+
+    flops = float(sum(e["count"] * (e["n"] ** 3 / 3 + 2 * e["n"] ** 2) for e in groups))
+    res = {"batch": batch, "n": f"uniform {nmin}..{nmax}", "distinct_sizes": len(groups), "nrhs": 1,
+           "streams": NSTREAMS}
+    for kind in ("fp64_forward", "fp64_backward", "fp64_backward_recompute", "fp32_backward"):
+        t_eager = time_steps(lambda: run(kind), steps, warmup)
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(s):
+            run(kind)
+            s.synchronize()
+            with torch.cuda.graph(graph, stream=s):
+                run(kind)
+        torch.cuda.current_stream().wait_stream(s)
+        t_graph = time_steps(graph.replay, steps, warmup)
+        res[kind] = {"instances_per_s_graph": batch / t_graph, "instances_per_s_eager": batch / t_eager,
+                     "ms_per_step_graph": t_graph * 1e3, "ms_per_step_eager": t_eager * 1e3}
+        if kind == "fp64_forward":
+            res[kind]["fp64_tflops_graph"] = flops / t_graph / 1e12
+        # check a sample of every 16th size against the FP64 reference adjoint
+        if kind != "fp64_forward":
+            worst = 0.0
+            for ent in groups[::16]:
+                o = ent["float32" if kind == "fp32_backward" else "float64"]
+                x = o["X"].logical(ent["count"])[0].astype(np.float64)
+                ab_ref, bb_ref = adjoint.posv_backward(np.tril(ent["a"][0].cpu().numpy()), x,
+                                                       ent["xb"][0].cpu().numpy())
+                worst = max(worst, np.abs(o["Ab"].logical(ent["count"])[0] - ab_ref).max() / np.abs(ab_ref).max(),
+                            np.abs(o["Bb"].logical(ent["count"])[0] - bb_ref).max() / np.abs(bb_ref).max())
+            res[kind]["max_rel_err_sample_vs_fp64_reference"] = float(worst)
+    return res
+
+
 def cpu_baseline(la, lb, abuf, bbuf, target_s):
     from oracle import oracle
 
@@ -413,10 +553,12 @@ def main():
 
     from blast_amd import capi
 
-    if args.only in ("gemm", "riccati", "mixed256"):
+    if args.only in ("gemm", "riccati", "mixed256", "adjoint", "convert"):
         res = {"gemm": lambda: gemm_workload(args.steps, args.warmup),
                "riccati": lambda: riccati_workload(args.steps, args.warmup, args.riccati_count),
-               "mixed256": lambda: mixed256_workload(args.steps, args.warmup)}[args.only]()
+               "mixed256": lambda: mixed256_workload(args.steps, args.warmup),
+               "adjoint": lambda: adjoint_workload(args.steps, args.warmup),
+               "convert": lambda: convert_workload(args.steps, args.warmup)}[args.only]()
         print(json.dumps({args.only: res}))
         return
     batch = args.batch
@@ -523,8 +665,10 @@ def main():
         return
     line["config"]["other_workloads"] = {
         "dgemm_64_batch4096": gemm_workload(args.steps, args.warmup),
+        "convert_n128_batch4096": convert_workload(args.steps, args.warmup),
         f"riccati_uneven_{args.riccati_count}_h10": riccati_workload(args.steps, args.warmup, args.riccati_count),
         "mixed_n256_nrhs8_batch2048": mixed256_workload(min(args.steps, 5), min(args.warmup, 1)),
+        "adjoint_spd_solve_n32_192_batch4096": adjoint_workload(min(args.steps, 5), min(args.warmup, 1)),
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(la, lb, abuf, bbuf, args.cpu_seconds)

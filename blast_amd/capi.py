@@ -72,6 +72,8 @@ def lib():
             getattr(L, f"blast_hip_{p}posv_backward_batched").argtypes = [vp, c_int, P, P, P, P, P, i64, i32p]
             getattr(L, f"blast_hip_{p}getrs_batched").argtypes = [vp, P, vp, i64, P, P, i64]
             getattr(L, f"blast_hip_{p}getrs_backward_batched").argtypes = [vp, P, vp, i64, P, P, P, P, i64]
+        L.blast_hip_dcopy_batched.argtypes = [vp, P, P, i64]
+        L.blast_hip_scopy_batched.argtypes = [vp, P, P, i64]
         L.blast_hip_mixed_work_elems.argtypes = [i64]
         L.blast_hip_mixed_work_elems.restype = i64
         L.blast_hip_dsposv_ws_batched.argtypes = [vp, c_int, c_int, ctypes.c_double, P, P, P, vp, i64, i32p, i32p]
@@ -211,6 +213,12 @@ def sposv(precision, A, B, X, batch, iter_, info=None, itmax=30, tol=0.0, stream
     name = "blast_hip_dsposv_batched"
     return check(name, lib().blast_hip_dsposv_batched(_stream(stream), int(precision), int(itmax), float(tol), _ref(A),
                                                       _ref(B), _ref(X), batch, _info_ptr(info), _info_ptr(iter_)))
+
+
+def copy(S, D, batch, stream=None):
+    """D := S (any two storage layouts of the same logical size)."""
+    name = f"blast_hip_{_prefix(D.layout.dtype)}copy_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), _ref(S), _ref(D), batch))
 
 
 def mixed_work_elems(n):

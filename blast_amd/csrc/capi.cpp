@@ -310,6 +310,16 @@ int sposv_entry(void* stream, int precision, int itmax, double tol, const blast_
                            Mat<double>::from(*X), batch, info, iter);
 }
 
+template <typename T>
+int convert_entry(void* stream, const blast_matrix_t* S, const blast_matrix_t* D, int64_t batch) {
+    if (!valid<T>(S, "S")) return -2;
+    if (!valid<T>(D, "D")) return -3;
+    if (!valid_batch(batch)) return -4;
+    if (S->rows != D->rows || S->cols != D->cols) { set_error("copy: Matrix sizes do not match"); return -3; }
+    (void)hipGetLastError();
+    return launch_convert<T>(hipStream_t(stream), Mat<T>::from(*S), Mat<T>::from(*D), batch);
+}
+
 // Shared checks of the mixed-precision and LU solve entry points.
 int solve_shapes(const char* who, const blast_matrix_t* A, const blast_matrix_t* B, const blast_matrix_t* X, int ia,
                  int ib, int ix) {
@@ -402,6 +412,13 @@ int blast_hip_dsposv_batched(void* s, int precision, int itmax, double tol, cons
                              const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch, int32_t* info,
                              int32_t* iter) {
     return sposv_entry(s, precision, itmax, tol, A, B, X, batch, info, iter);
+}
+
+int blast_hip_dcopy_batched(void* s, const blast_matrix_t* S, const blast_matrix_t* D, int64_t batch) {
+    return convert_entry<double>(s, S, D, batch);
+}
+int blast_hip_scopy_batched(void* s, const blast_matrix_t* S, const blast_matrix_t* D, int64_t batch) {
+    return convert_entry<float>(s, S, D, batch);
 }
 
 int64_t blast_hip_mixed_work_elems(int64_t n) { return (n >= 1 && n <= 256) ? wg_work_elems(int(n)) : 0; }

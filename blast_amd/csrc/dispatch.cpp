@@ -28,6 +28,13 @@ int dispatch_trmm(hipStream_t s, int side_right, T alpha, Mat<T> A, Mat<T> B, Ma
 template <typename T>
 int dispatch_trsm(hipStream_t s, int side_right, int upper, int unit, T alpha, Mat<T> A, Mat<T> B, Mat<T> X,
                   int64_t batch) {
+    if (algorithm() == BLAST_HIP_ALGO_AUTO) {
+        // X A = alpha B  <=>  A^T X^T = alpha B^T (A^T has the other triangle)
+        int const r = side_right ? launch_trsm_tiles<T>(s, !upper, unit, alpha, transposed(A), transposed(B),
+                                                        transposed(X), batch)
+                                 : launch_trsm_tiles<T>(s, upper, unit, alpha, A, B, X, batch);
+        if (r != 1) return r;
+    }
     return launch_trsm_ref<T>(s, side_right, upper, unit, alpha, A, B, X, batch);
 }
 

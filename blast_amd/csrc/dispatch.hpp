@@ -23,6 +23,10 @@ template <typename T> int launch_trmm_tiles(hipStream_t, int, T, Mat<T>, Mat<T>,
 template <typename T> int launch_upper_times_lower(hipStream_t, T, Mat<T>, Mat<T>, Mat<T>, int64_t);
 template <typename T> int launch_gemm_overwrite(hipStream_t, T, Mat<T>, Mat<T>, Mat<T>, int64_t);
 
+// trsm_tiles.hip: left solve, square A, n <= 256; returns 1 when not covered
+template <typename T>
+int launch_trsm_tiles(hipStream_t, int upper, int unit, T alpha, Mat<T> A, Mat<T> B, Mat<T> X, int64_t batch);
+
 // adjoint.hip
 template <typename T>
 int gemm_backward(hipStream_t, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> Db, Mat<T> Ab, Mat<T> Bb, Mat<T> Cb,
@@ -53,6 +57,9 @@ int launch_sposv_ir(hipStream_t, int bf16, Mat<double> A, Mat<double> B, Mat<dou
 int launch_fill_i32(hipStream_t, int32_t* p, int32_t v, int64_t count);
 int dispatch_dsposv(hipStream_t, int bf16, int itmax, double tol, Mat<double> A, Mat<double> B, Mat<double> X,
                     int64_t batch, int32_t* info, int32_t* iter);
+
+// convert.hip: D := S between any two storage layouts (same logical size)
+template <typename T> int launch_convert(hipStream_t, Mat<T> S, Mat<T> D, int64_t batch);
 
 // wg_solve.hip: one workgroup per instance, square n <= 256, nrhs <= 8.
 bool wg_covers(int n, int nrhs);

@@ -238,4 +238,49 @@ __device__ int diag_factor(T* sD, T* sW, int lane) {
     return bad;
 }
 
+// Inverse of the 16x16 triangular tile in sA (T[m][k] at k*17 + m; lower,
+// or upper when `upper`; unit diagonal when `unit`: the stored diagonal is
+// not read) into sW, same layout, same triangle.  One wavefront; may run in
+// place (sW == sA).  Row-by-row substitution as in diag_factor: lane i holds
+// row i of the lower-triangular form, rows finalised at step j are
+// broadcast by DPP from lane j.
+template <typename T>
+__device__ void tri_inverse(const T* sA, T* sW, int lane, bool upper, bool unit) {
+    int const i = lane & 15;
+    T t[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) t[c] = upper ? sA[i * kTileStride + c] : sA[c * kTileStride + i];
+    T di = T(1);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) di = (c == i) ? t[c] : di;
+    T const myr = unit ? T(1) : T(1) / di;
+    T w[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) w[c] = T(0);
+    static_for<0, 16>([&](auto J) __attribute__((always_inline)) {
+        constexpr int j = decltype(J)::value;
+        static_for<0, j>([&](auto C) __attribute__((always_inline)) {
+            constexpr int c = decltype(C)::value;
+            w[c] = (i == j) ? -w[c] * myr : w[c];
+        });
+        w[j] = (i == j) ? myr : w[j];
+        T const f = (i > j) ? t[j] : T(0);
+        static_for<0, j + 1>([&](auto C) __attribute__((always_inline)) {
+            constexpr int c = decltype(C)::value;
+            w[c] = __builtin_fma(f, row_bcast<j>(w[c]), w[c]);
+        });
+        __builtin_amdgcn_sched_barrier(0);
+    });
+    if (lane < 16) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            T const v = (c <= i) ? w[c] : T(0);  // W_lower[i][c]
+            if (upper)
+                sW[i * kTileStride + c] = v;  // W = W_lower^T
+            else
+                sW[c * kTileStride + i] = v;
+        }
+    }
+}
+
 }  // namespace blast_hip

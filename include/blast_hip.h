@@ -210,6 +210,15 @@ int blast_hip_sgetrs_backward_batched(void* stream, const blast_matrix_t* LU, co
                                       int64_t ipiv_stride, const blast_matrix_t* X, const blast_matrix_t* Xbar,
                                       const blast_matrix_t* Abar, const blast_matrix_t* Bbar, int64_t batch);
 
+/* D := S element by element between any two storage layouts of the same
+ * logical size (dense or panel, either transposed view): the batched device
+ * form of assigning a panel matrix to a dense one and back
+ * (include/blast/math/expressions/PanelMatrix.hpp:81-167 and :170-239).
+ * Only the logical rows x cols elements of D are written; S and D must not
+ * overlap unless they are the same descriptor. */
+int blast_hip_dcopy_batched(void* stream, const blast_matrix_t* S, const blast_matrix_t* D, int64_t batch);
+int blast_hip_scopy_batched(void* stream, const blast_matrix_t* S, const blast_matrix_t* D, int64_t batch);
+
 /* ---- Solves for 128 < n <= 256 (and any n <= 256), nrhs <= 8 ----
  * BLAST has no solve driver; these compose its getrf (dense/Getrf.hpp:52-119)
  * or potrf (dense/Potrf.hpp:66-106) with trsm (dense/Trsm.hpp:17-66) and

@@ -209,3 +209,37 @@ def test_trmm_tiles(side_right, m, n):
     capi.trmm(side_right, not side_right, 0.75, dev(la, la.pack(a)), dev(lb, lb.pack(b)), C, batch)
     ref = 0.75 * (b @ np.tril(a) if side_right else np.triu(a) @ b)
     np.testing.assert_allclose(C.logical(batch), ref, rtol=0, atol=1e-13 * na)
+
+
+# ------------------------------------------------------------------ trsm on MFMA tiles
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 2e-5)])
+@pytest.mark.parametrize("side_right,upper,unit", [(0, 0, 0), (0, 1, 0), (0, 0, 1), (0, 1, 1), (1, 1, 0), (1, 0, 1),
+                                                   (1, 0, 0)])
+@pytest.mark.parametrize("m,n", [(1, 1), (17, 3), (64, 33), (130, 5), (256, 8), (5, 200)])
+@pytest.mark.parametrize("trans_a", [False, True])
+def test_trsm_tiles_vs_oracle(dtype, tol, side_right, upper, unit, m, n, trans_a):
+    rng = np.random.default_rng(m * 7 + n + 3 * side_right + upper)
+    batch = 3
+    na = n if side_right else m
+    if na > 256:
+        return
+    # well conditioned with or without the unit diagonal: off-diagonal entries U(-1,1)/n
+    a = rng.uniform(-1, 1, (batch, na, na)) / na + np.eye(na) * (1 + rng.uniform(0, 1, (batch, 1, na)))
+    a = np.triu(a) if upper else np.tril(a)
+    a += np.triu(rng.standard_normal((batch, na, na)), 1) * 50 if not upper else np.tril(
+        rng.standard_normal((batch, na, na)), -1) * 50  # garbage in the other triangle: must not be read
+    b = rng.uniform(-1, 1, (batch, m, n))
+    la = (Layout.make(na, na, dtype, panel=True).t() if trans_a else Layout.make(na, na, dtype, panel=True))
+    lb = Layout.make(m, n, dtype, panel=False)
+    ah, bh = la.pack(a.astype(dtype)), lb.pack(b.astype(dtype))
+    side = "right" if side_right else "left"
+    xh = np.zeros_like(bh)
+    oracle.trsm(side, not upper, unit, 1.5, la, ah, lb, bh, lb, xh, batch)
+    X = capi.Operand.empty(lb, batch)
+    capi.trsm(side_right, upper, unit, 1.5, dev(la, ah), dev(lb, bh), X, batch)
+    x, x_ref = X.logical(batch), lb.unpack(xh, batch)
+    np.testing.assert_allclose(x, x_ref, rtol=0, atol=tol * max(1.0, np.abs(x_ref).max()))
+    # X may alias B
+    B = dev(lb, bh)
+    capi.trsm(side_right, upper, unit, 1.5, dev(la, ah), B, B, batch)
+    np.testing.assert_allclose(B.logical(batch), x_ref, rtol=0, atol=tol * max(1.0, np.abs(x_ref).max()))
