@@ -6,8 +6,14 @@ one batched call over the whole batch with every input already in HBM:
 
   plain FP64  blast_hip_dposv_batched: A (lower) in, L (lower) and X out,
               the LAPACK dposv contract;
-  mixed + IR  blast_hip_dsposv_batched (when built): factor in lower
-              precision, refine in FP64, X out (LAPACK dsposv contract).
+  mixed + IR  blast_hip_dsposv_batched: factor in FP32 (or BF16 products),
+              refine in FP64, X out (LAPACK dsposv contract).
+
+`value` is the mixed + IR rate when its measured backward error meets the
+1e-12 bar (else the plain FP64 rate); both paths are reported.  The line also
+carries the other workloads (dgemm 64^3, storage conversion, the uneven
+Riccati recursion, n = 256 mixed precision, adjoints) under
+config.other_workloads.
 
 Prints one JSON line.  Time is measured with HIP events on the stream the
 kernels run on, over --steps steps after --warmup steps.  The CPU baseline is
@@ -109,14 +115,12 @@ def profiled_traffic(kernel):
     summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
     import re
 
-    path = os.path.join(ROOT, "profiles", "posv_wave_n128_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", "posv_n128_hbm_traffic.md")
     try:
         text = open(path).read()
     except OSError:
         return None
-    if kernel not in text:
-        return None
-    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", text)
+    m = re.search(r"traffic_bytes_per_launch " + re.escape(kernel) + r":\s*(\d+)", text)
     return int(m.group(1)) if m else None
 
 
@@ -627,7 +631,29 @@ def main():
         value, t_step, dtype = plain_rate, t_plain, "f64"
         headline = "plain_fp64"
 
-    achieved = BYTES_PLAIN * batch / t_plain / 1e9
+    # roofline of the kernel behind `value`; the plain FP64 kernel's beside it
+    t_mixed = mixed["ms_per_step"] / 1e3
+    roof_plain = {
+        "bound": "hbm", "kernel": "posv_wave_kernel<8, true>",
+        "achieved": BYTES_PLAIN * batch / t_plain / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+        "frac": BYTES_PLAIN * batch / t_plain / 1e9 / PEAK_HBM_GBS,
+        "traffic": profiled_traffic("posv_wave_kernel<8, true>"), "algorithmic_bytes_per_instance": BYTES_PLAIN,
+        "fp64_tflops": FLOPS_PER_INSTANCE * batch / t_plain / 1e12,
+        "fp64_frac_of_peak": FLOPS_PER_INSTANCE * batch / t_plain / 1e12 / PEAK_FP64_TFLOPS,
+    }
+    if headline == "mixed_fp32_ir":
+        roofline = {
+            "bound": "hbm", "kernel": "sposv_ir_kernel<8, true, false>",
+            "achieved": BYTES_SOLVE_ONLY * batch / t_mixed / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": BYTES_SOLVE_ONLY * batch / t_mixed / 1e9 / PEAK_HBM_GBS,
+            "traffic": profiled_traffic("sposv_ir_kernel<8, true, false>"),
+            "algorithmic_bytes_per_instance": BYTES_SOLVE_ONLY,
+            "timed": "one step = the mixed kernel + the FP64 fallback launch (exits at once when nothing fell back)",
+            "plain_fp64": roof_plain,
+        }
+    else:
+        roofline = roof_plain
+
     line = {
         "metric": "batched potrf+trsm solves/s @n=128: mixed+IR vs plain FP64, both <=1e-12 bwd err",
         "value": value,
@@ -647,18 +673,7 @@ def main():
             "headline_path": headline,
             "paths": result,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS,
-            "traffic": profiled_traffic("posv_wave_kernel<8, true>"),
-            "kernel": "posv_wave_kernel<8, true>",
-            "algorithmic_bytes_per_instance": BYTES_PLAIN,
-            "fp64_tflops": FLOPS_PER_INSTANCE * batch / t_plain / 1e12,
-            "fp64_frac_of_peak": FLOPS_PER_INSTANCE * batch / t_plain / 1e12 / PEAK_FP64_TFLOPS,
-        },
+        "roofline": roofline,
     }
     if args.only == "posv":
         print(json.dumps(line))
