@@ -64,16 +64,23 @@ __device__ void wg_gemm(double alpha, DV A, DV B, double beta, DV C, DV D, int M
             d4 acc[TN];
 #pragma unroll
             for (int t = 0; t < TN; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
-            for (int q = 0; q < KQ; ++q) {
-                int const k = 4 * q + g;
-                bool const ina = row < M && k < K && (MODE != kUpperA || k >= row);
-                double const av = ina ? A(row, k) : 0.0;
+            for (int q0 = 0; q0 < KQ; q0 += 4) {  // four K-chunks of loads in flight
+                double av[4], bv[4][TN];
 #pragma unroll
-                for (int t = 0; t < TN; ++t) {
-                    int const col = 16 * (T0 + t) + m;
-                    double const bv = (col < N && k < K) ? B(k, col) : 0.0;
-                    acc[t] = Mf::mma(av, bv, acc[t]);
+                for (int u = 0; u < 4; ++u) {
+                    int const k = 4 * (q0 + u) + g;
+                    bool const ina = row < M && k < K && (MODE != kUpperA || k >= row);
+                    av[u] = ina ? A(row, k) : 0.0;
+#pragma unroll
+                    for (int t = 0; t < TN; ++t) {
+                        int const col = 16 * (T0 + t) + m;
+                        bv[u][t] = (col < N && k < K) ? B(k, col) : 0.0;
+                    }
                 }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int t = 0; t < TN; ++t) acc[t] = Mf::mma(av[u], bv[u][t], acc[t]);
             }
 #pragma unroll
             for (int t = 0; t < TN; ++t)
@@ -89,62 +96,61 @@ __device__ void wg_gemm(double alpha, DV A, DV B, double beta, DV C, DV D, int M
 }
 
 // In-place lower Cholesky of the n x n matrix a (lower triangle read and
-// written), blocked by 16 columns: the 16x16 diagonal block is factored in
-// LDS, the panel below it solved row by row, the trailing matrix updated on
-// MFMA tiles.  Returns the first non-positive pivot column (1-based) or 0.
+// written), blocked by 16 columns: wavefront 0 factors the 16x16 diagonal
+// block in LDS and forms W = L_kk^{-1} (diag_factor, no barriers inside), the
+// panel below is L_ik = A_ik W^T on MFMA, the trailing matrix is updated on
+// MFMA tiles.  Returns the first non-positive pivot column (1-based) or 0
+// (uniform across the workgroup).
 template <int TN>
-__device__ int wg_potrf(DV a, int n, double* sB) {
-    int bad = 0;
+__device__ int wg_potrf(DV a, int n, double* sB, double* sW, int* sBad) {
+    using Mf = Mfma<double>;
+    int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) *sBad = 0;
     for (int k0 = 0; k0 < n; k0 += 16) {
         int const kb = min(16, n - k0);
-        // diagonal block -> LDS (17-stride), factor, write back
-        for (int e = threadIdx.x; e < kb * kb; e += kT) {
-            int const i = e % kb, j = e / kb;
-            sB[j * 17 + i] = i >= j ? a(k0 + i, k0 + j) : 0.0;
+        // diagonal block -> LDS (T[m][k] at k*17+m), identity beyond kb
+        for (int e = threadIdx.x; e < 256; e += kT) {
+            int const i = e & 15, j = e >> 4;
+            sB[j * kTileStride + i] = (i < kb && j < kb) ? (i >= j ? a(k0 + i, k0 + j) : 0.0) : (i == j ? 1.0 : 0.0);
         }
         __syncthreads();
-        for (int c = 0; c < kb; ++c) {
-            double const d = sB[c * 17 + c];
-            if (!(d > 0.0) && bad == 0) bad = k0 + c + 1;
-            double const s = __builtin_sqrt(d);
-            __syncthreads();
-            if (threadIdx.x < kb && int(threadIdx.x) >= c) sB[c * 17 + threadIdx.x] = int(threadIdx.x) == c ? s : sB[c * 17 + threadIdx.x] / s;
-            __syncthreads();
-            for (int e = threadIdx.x; e < kb * kb; e += kT) {
-                int const i = e % kb, j = e / kb;
-                if (j > c && i >= j) sB[j * 17 + i] = __builtin_fma(-sB[c * 17 + i], sB[c * 17 + j], sB[j * 17 + i]);
-            }
-            __syncthreads();
+        if (w == 0) {
+            int const bk = diag_factor<double>(sB, sW, lane);
+            if (lane == 0 && bk && bk <= kb && *sBad == 0) *sBad = k0 + bk;
         }
+        __syncthreads();
         for (int e = threadIdx.x; e < kb * kb; e += kT) {
             int const i = e % kb, j = e / kb;
-            if (i >= j) a(k0 + i, k0 + j) = sB[j * 17 + i];
+            if (i >= j) a(k0 + i, k0 + j) = sB[j * kTileStride + i];
         }
-        // panel: rows below solve x L_kk^T = a_row
-        for (int i = k0 + kb + threadIdx.x; i < n; i += kT) {
-            double x[16];
+        // panel: L_ik = A_ik W^T, 16-row tiles over the wavefronts
+        int const r0 = k0 + kb, rest = n - r0;
+        for (int R = w; 16 * R < rest; R += kT / 64) {
+            int const row = r0 + 16 * R + (lane & 15);
+            double av[4];
 #pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                if (c < kb) {
-                    double v = a(i, k0 + c);
+            for (int q = 0; q < 4; ++q) {
+                int const k = 4 * q + (lane >> 4);
+                av[q] = (row < n && k < kb) ? a(row, k0 + k) : 0.0;
+            }
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                    for (int j = 0; j < 16; ++j)
-                        if (j < c) v = __builtin_fma(-x[j], sB[j * 17 + c], v);
-                    x[c] = v / sB[c * 17 + c];
-                    a(i, k0 + c) = x[c];
-                }
-                __builtin_amdgcn_sched_barrier(0);
+            for (int q = 0; q < 4; ++q) acc = Mf::mma(av[q], op_read(sW, q, lane), acc);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int const i = r0 + 16 * R + Mf::row(lane, r), j = acc_col(lane);
+                if (i < n && j < kb) a(i, k0 + j) = acc[r];
             }
         }
         __syncthreads();
         // trailing update, lower triangle: A22 -= L21 L21^T
-        int const rest = n - k0 - kb;
         if (rest > 0) {
-            DV const l21 = a.sub(k0 + kb, k0), a22 = a.sub(k0 + kb, k0 + kb);
+            DV const l21 = a.sub(r0, k0), a22 = a.sub(r0, r0);
             wg_gemm<TN, kSyrkAdd>(-1.0, l21, l21.tr(), 1.0, a22, a22, rest, rest, kb);
         }
     }
-    return bad;
+    __syncthreads();
+    return *sBad;
 }
 
 // Solve U X = B in place for upper-triangular U (here U = Ls^T as a
@@ -164,7 +170,7 @@ __device__ void wg_trsm_upper(DV U, DV Bm, int n, int nrhs) {
 
 template <int TN>
 __device__ void riccati_instance(RiccatiInstance const& ins, int horizon, const double* in, double* out, double* work,
-                                 double* sB, int32_t* info_out) {
+                                 double* sB, double* sW, int* sBad, int32_t* info_out) {
     int const nx = ins.nx, nu = ins.nu, nt = nx + nu;
     DV const A{const_cast<double*>(in) + ins.a, nx, false};
     DV const B{const_cast<double*>(in) + ins.b, nx, false};
@@ -185,7 +191,7 @@ __device__ void riccati_instance(RiccatiInstance const& ins, int horizon, const 
         Lcur(i, j) = i >= j ? Q(i, j) : 0.0;
     }
     __syncthreads();
-    int b0 = wg_potrf<TN>(Lcur, nx, sB);
+    int b0 = wg_potrf<TN>(Lcur, nx, sB, sW, sBad);
     if (b0) bad = b0;
     for (int k = horizon - 1; k >= 0; --k) {
         // M = L^T [B A]
@@ -197,7 +203,7 @@ __device__ void riccati_instance(RiccatiInstance const& ins, int horizon, const 
         }
         __syncthreads();
         wg_gemm<TN, kSyrkAdd>(1.0, Mm.tr(), Mm, 1.0, Lam, Lam, nt, nt, nx);
-        int const bk = wg_potrf<TN>(Lam, nt, sB);
+        int const bk = wg_potrf<TN>(Lam, nt, sB, sW, sBad);
         if (bk && bad == 0) bad = bk;
         // K_k = -Ls^{-T} Lks^T
         DV const Kk{out + ins.k + int64_t(k) * nu * nx, nu, false};
@@ -217,12 +223,13 @@ __device__ void riccati_instance(RiccatiInstance const& ins, int horizon, const 
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kT) void riccati_persistent_kernel(const RiccatiInstance* inst, int64_t start,
+__global__ __launch_bounds__(kT, 2) void riccati_persistent_kernel(const RiccatiInstance* inst, int64_t start,
                                                                 int64_t count, int horizon, const double* in,
                                                                 double* out, double* work, int32_t* queue,
                                                                 RiccatiTuning tuning, int32_t* info) {
-    __shared__ double sB[16 * 17];
-    __shared__ int sIdx;
+    __shared__ double sB[kTileElems];
+    __shared__ double sW[kTileElems];
+    __shared__ int sIdx, sBad;
     for (;;) {
         if (threadIdx.x == 0) sIdx = atomicAdd(&queue[0], 1);
         __syncthreads();
@@ -234,11 +241,11 @@ __global__ __launch_bounds__(kT) void riccati_persistent_kernel(const RiccatiIns
         int const tn = tuning.tn[min(ins.nx / 16, 15)];
         int32_t* inf = info ? info + idx : nullptr;
         if (tn == 1)
-            riccati_instance<1>(ins, horizon, in, out, work, sB, inf);
+            riccati_instance<1>(ins, horizon, in, out, work, sB, sW, &sBad, inf);
         else if (tn == 2)
-            riccati_instance<2>(ins, horizon, in, out, work, sB, inf);
+            riccati_instance<2>(ins, horizon, in, out, work, sB, sW, &sBad, inf);
         else
-            riccati_instance<4>(ins, horizon, in, out, work, sB, inf);
+            riccati_instance<4>(ins, horizon, in, out, work, sB, sW, &sBad, inf);
     }
 }
 
