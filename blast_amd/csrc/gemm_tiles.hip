@@ -43,17 +43,27 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(T alpha, Mat<T> A, Mat<T
             acc_t acc[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) acc[t] = acc_t{};
-            for (int q = 0; q < KQ; ++q) {
-                int const k = 4 * q + g;
-                bool const ina = row < Mr && k < K && ((MODE != kTrmmLeftUpper && MODE != kUpperTimesLower) || k >= row);
-                T const av = ina ? a(row, k) : T(0);
+            constexpr int U = 4;  // K-chunks of loads in flight (8 costs occupancy)
+            for (int q0 = 0; q0 < KQ; q0 += U) {
+                T av[U], bv[U][4];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    int const col = 16 * (T0 + t) + m;  // B-operand column of this lane
-                    bool const inb = col < Nc && k < K && ((MODE != kTrmmRightLower && MODE != kUpperTimesLower) || k >= col);
-                    T const bv = inb ? bb(k, col) : T(0);
-                    acc[t] = M::mma(av, bv, acc[t]);
+                for (int u = 0; u < U; ++u) {
+                    int const k = 4 * (q0 + u) + g;
+                    bool const ina =
+                        row < Mr && k < K && ((MODE != kTrmmLeftUpper && MODE != kUpperTimesLower) || k >= row);
+                    av[u] = ina ? a(row, k) : T(0);
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        int const col = 16 * (T0 + t) + m;  // B-operand column of this lane
+                        bool const inb = col < Nc && k < K &&
+                                         ((MODE != kTrmmRightLower && MODE != kUpperTimesLower) || k >= col);
+                        bv[u][t] = inb ? bb(k, col) : T(0);
+                    }
                 }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) acc[t] = M::mma(av[u], bv[u][t], acc[t]);
             }
 #pragma unroll
             for (int t = 0; t < 4; ++t)
