@@ -22,12 +22,15 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "../../blast_hip.h"
 
@@ -353,6 +356,108 @@ inline void sposv(int precision, hip::Batch<double> A, hip::Batch<double const> 
                   hipStream_t s = nullptr) {
     hip::check(blast_hip_dsposv_batched(s, precision, itmax, tol, &A.desc, &B.desc, &X.desc, X.count, info, iter));
 }
+
+// Storage conversion D := S between any two layouts of the same logical size:
+// the batched form of assigning a panel matrix to a dense one and back
+// (expressions/PanelMatrix.hpp:81-167, :170-239).
+template <typename T>
+void assign(hip::Batch<T const> S, hip::Batch<T> D, hipStream_t s = nullptr) {
+    if constexpr (std::is_same_v<T, double>)
+        hip::check(blast_hip_dcopy_batched(s, &S.desc, &D.desc, D.count));
+    else
+        hip::check(blast_hip_scopy_batched(s, &S.desc, &D.desc, D.count));
+}
+
+// getrf + getrs (LAPACK dgesv): A holds its LU afterwards, ipiv its pivots.
+inline void gesv(hip::Batch<double> A, std::int64_t* ipiv, std::int64_t ipiv_stride, hip::Batch<double const> B,
+                 hip::Batch<double> X, std::int32_t* info = nullptr, hipStream_t s = nullptr) {
+    hip::check(blast_hip_dgesv_batched(s, &A.desc, ipiv, ipiv_stride, &B.desc, &X.desc, X.count, info));
+}
+
+// FP32 factor buffer (floats per instance) of the n <= 256 mixed-precision solves.
+inline std::int64_t mixedWorkElems(std::int64_t n) { return blast_hip_mixed_work_elems(n); }
+
+// Mixed precision Cholesky solve for n <= 256, nrhs <= 8 with the FP32 factor
+// in `work` (LAPACK dsposv with SWORK).
+inline void sposv(int precision, hip::Batch<double> A, hip::Batch<double const> B, hip::Batch<double> X, float* work,
+                  std::int32_t* iter, std::int32_t* info = nullptr, int itmax = 30, double tol = 0.0,
+                  hipStream_t s = nullptr) {
+    hip::check(
+        blast_hip_dsposv_ws_batched(s, precision, itmax, tol, &A.desc, &B.desc, &X.desc, work, X.count, info, iter));
+}
+
+// Mixed precision LU solve with FP64 refinement (LAPACK dsgesv).
+inline void sgesv(int precision, hip::Batch<double> A, std::int64_t* ipiv, std::int64_t ipiv_stride,
+                  hip::Batch<double const> B, hip::Batch<double> X, float* work, std::int32_t* iter,
+                  std::int32_t* info = nullptr, int itmax = 30, double tol = 0.0, hipStream_t s = nullptr) {
+    hip::check(blast_hip_dsgesv_batched(s, precision, itmax, tol, &A.desc, ipiv, ipiv_stride, &B.desc, &X.desc, work,
+                                        X.count, info, iter));
+}
+
+namespace hip {
+
+// Uneven batch of Riccati recursions (LQR backward passes): owns the plan,
+// the device buffers and the queue; setInputs() takes per-instance A, B, Q,
+// R (dense column-major, ld = rows) in one host array laid out by the plan.
+class RiccatiBatch {
+public:
+    RiccatiBatch(std::vector<std::int32_t> const& nx, std::vector<std::int32_t> const& nu, int horizon)
+        : horizon_(horizon), inst_(nx.size()), order_(nx.size()) {
+        std::int64_t ni = 0, no = 0, nw = 0;
+        check(blast_hip_riccati_plan(std::int64_t(nx.size()), nx.data(), nu.data(), horizon, inst_.data(),
+                                     order_.data(), &ni, &no, &nw));
+        in_ = std::make_unique<DeviceArray<double>>(std::size_t(ni));
+        out_ = std::make_unique<DeviceArray<double>>(std::size_t(no));
+        work_ = std::make_unique<DeviceArray<double>>(std::size_t(nw));
+        dinst_ = std::make_unique<DeviceArray<blast_riccati_instance_t>>(inst_.size());
+        queue_ = std::make_unique<DeviceArray<std::int32_t>>(inst_.size() + 1);
+        info_ = std::make_unique<DeviceArray<std::int32_t>>(inst_.size());
+        std::vector<std::int32_t> q(inst_.size() + 1, 0);
+        std::copy(order_.begin(), order_.end(), q.begin() + 1);
+        check_hip(hipMemcpy(dinst_->get(), inst_.data(), inst_.size() * sizeof(blast_riccati_instance_t),
+                            hipMemcpyHostToDevice));
+        check_hip(hipMemcpy(queue_->get(), q.data(), q.size() * sizeof(std::int32_t), hipMemcpyHostToDevice));
+        nin_ = std::size_t(ni), nout_ = std::size_t(no);
+    }
+    std::vector<blast_riccati_instance_t> const& instances() const noexcept { return inst_; }
+    std::size_t inputSize() const noexcept { return nin_; }
+    std::size_t outputSize() const noexcept { return nout_; }
+    void setInputs(std::vector<double> const& host) {
+        if (host.size() != nin_) throw std::invalid_argument("RiccatiBatch: input size does not match the plan");
+        check_hip(hipMemcpy(in_->get(), host.data(), nin_ * sizeof(double), hipMemcpyHostToDevice));
+    }
+    // One persistent launch; no allocation, no memset: capturable in a hipGraph.
+    void run(hipStream_t s = nullptr) {
+        check(blast_hip_driccati_batched(s, horizon_, dinst_->get(), std::int64_t(inst_.size()), in_->get(),
+                                         out_->get(), work_->get(), queue_->get(), info_->get()));
+    }
+    void autotune(int reps = 3, hipStream_t s = nullptr) {
+        check(blast_hip_riccati_autotune(s, horizon_, dinst_->get(), inst_.data(), std::int64_t(inst_.size()),
+                                         in_->get(), out_->get(), work_->get(), queue_->get(), order_.data(), reps,
+                                         nullptr));
+    }
+    std::vector<double> outputs() const {
+        std::vector<double> host(nout_);
+        check_hip(hipMemcpy(host.data(), out_->get(), nout_ * sizeof(double), hipMemcpyDeviceToHost));
+        return host;
+    }
+    std::vector<std::int32_t> info() const {
+        std::vector<std::int32_t> host(inst_.size());
+        check_hip(hipMemcpy(host.data(), info_->get(), host.size() * sizeof(std::int32_t), hipMemcpyDeviceToHost));
+        return host;
+    }
+
+private:
+    int horizon_;
+    std::vector<blast_riccati_instance_t> inst_;
+    std::vector<std::int32_t> order_;
+    std::size_t nin_ = 0, nout_ = 0;
+    std::unique_ptr<DeviceArray<double>> in_, out_, work_;
+    std::unique_ptr<DeviceArray<blast_riccati_instance_t>> dinst_;
+    std::unique_ptr<DeviceArray<std::int32_t>> queue_, info_;
+};
+
+}  // namespace hip
 
 // ------------------------------------------------------------------ single-instance drop-ins (host memory)
 namespace detail {

@@ -193,6 +193,92 @@ int main() {
         }
         CHECK(worst <= 1e-12);
     }
+    // Storage conversion on the device: panel -> row-major dense, bit-exact.
+    {
+        std::size_t const m = 37, n = 21, batch = 5;
+        blast::DynamicPanelMatrix<double> P(m, n);
+        blast::DynamicMatrix<double, blast::rowMajor> R(m, n);
+        std::size_t const sp = blast::nextMultiple(P.storageSize(), 8), sr = blast::nextMultiple(R.storageSize(), 8);
+        std::vector<double> hp(batch * sp), hr(batch * sr, -1.0);
+        for (std::size_t k = 0; k < batch; ++k) {
+            fill_dyadic(P, g);
+            std::memcpy(&hp[k * sp], P.data(), P.storageSize() * 8);
+        }
+        blast::hip::DeviceArray<double> dp(hp.size()), dr(hr.size());
+        blast::hip::check_hip(hipMemcpy(dp.get(), hp.data(), hp.size() * 8, hipMemcpyHostToDevice));
+        auto S = blast::hip::Batch<double>::like(P, dp.get(), batch);
+        auto D = blast::hip::Batch<double>::like(R, dr.get(), batch);
+        blast::assign<double>(S, D);
+        blast::hip::check_hip(hipMemcpy(hr.data(), dr.get(), hr.size() * 8, hipMemcpyDeviceToHost));
+        for (std::size_t k = 0; k < batch; ++k) {
+            std::memcpy(P.data(), &hp[k * sp], P.storageSize() * 8);
+            std::memcpy(R.data(), &hr[k * sr], R.storageSize() * 8);
+            for (std::size_t i = 0; i < m; ++i)
+                for (std::size_t j = 0; j < n; ++j) CHECK(R(i, j) == P(i, j));
+        }
+    }
+    // Batched LU solve (gesv) on 16 instances of n = 40.
+    {
+        std::size_t const n = 40, batch = 16;
+        blast::DynamicMatrix<double> protoA(n, n), protoB(n, 1);
+        std::size_t const sa = blast::nextMultiple(protoA.storageSize(), 8), sb = blast::nextMultiple(protoB.storageSize(), 8);
+        std::vector<double> ha(batch * sa), hb(batch * sb), hx(batch * sb);
+        std::uniform_real_distribution<double> u(-1, 1);
+        for (auto& v : ha) v = u(g);
+        for (auto& v : hb) v = u(g);
+        for (std::size_t k = 0; k < batch; ++k)
+            for (std::size_t i = 0; i < n; ++i) ha[k * sa + i + i * n] += 4.0;  // well conditioned
+        std::vector<double> ha0 = ha;
+        blast::hip::DeviceArray<double> da(ha.size()), db(hb.size()), dx(hb.size());
+        blast::hip::DeviceArray<std::int64_t> piv(batch * n);
+        blast::hip::check_hip(hipMemcpy(da.get(), ha.data(), ha.size() * 8, hipMemcpyHostToDevice));
+        blast::hip::check_hip(hipMemcpy(db.get(), hb.data(), hb.size() * 8, hipMemcpyHostToDevice));
+        auto A = blast::hip::Batch<double>::like(protoA, da.get(), batch);
+        auto B = blast::hip::Batch<double>::like(protoB, db.get(), batch);
+        auto X = blast::hip::Batch<double>::like(protoB, dx.get(), batch);
+        blast::gesv(A, piv.get(), std::int64_t(n), B, X);
+        blast::hip::check_hip(hipMemcpy(hx.data(), dx.get(), hx.size() * 8, hipMemcpyDeviceToHost));
+        double worst = 0.0;
+        for (std::size_t k = 0; k < batch; ++k)
+            for (std::size_t i = 0; i < n; ++i) {
+                double s = 0.0;
+                for (std::size_t q = 0; q < n; ++q) s += ha0[k * sa + i + q * n] * hx[k * sb + q];
+                worst = std::fmax(worst, std::fabs(s - hb[k * sb + i]));
+            }
+        CHECK(worst <= 1e-11);
+    }
+    // Uneven Riccati batch: scalar instances (nx = nu = 1) checked against the
+    // closed form of the scalar recursion.
+    {
+        int const horizon = 5;
+        std::vector<std::int32_t> nx = {1, 1, 3}, nu = {1, 1, 2};
+        blast::hip::RiccatiBatch rb(nx, nu, horizon);
+        std::vector<double> in(rb.inputSize(), 0.0);
+        double const av[2] = {1.3, 0.6}, bv[2] = {0.7, 1.1}, qv[2] = {2.0, 1.0}, rv[2] = {0.5, 3.0};
+        auto const& ins = rb.instances();
+        for (int k = 0; k < 2; ++k) {
+            in[ins[k].a] = av[k], in[ins[k].b] = bv[k], in[ins[k].q] = qv[k], in[ins[k].r] = rv[k];
+        }
+        for (int i = 0; i < 3; ++i) in[ins[2].q + i * 3 + i] = 1.0, in[ins[2].a + i * 3 + i] = 0.5;  // Q = I, A = 0.5 I
+        for (int i = 0; i < 2; ++i) in[ins[2].r + i * 2 + i] = 1.0, in[ins[2].b + i * 3 + i] = 1.0;
+        rb.setInputs(in);
+        rb.run();
+        blast::hip::check_hip(hipDeviceSynchronize());
+        auto out = rb.outputs();
+        auto info = rb.info();
+        for (int k = 0; k < 2; ++k) {
+            double p = qv[k];
+            double kk = 0.0;
+            for (int s = horizon - 1; s >= 0; --s) {
+                kk = -av[k] * bv[k] * p / (rv[k] + bv[k] * bv[k] * p);
+                CHECK(std::fabs(out[ins[k].k + s] - kk) <= 1e-14 * std::fabs(kk));
+                p = qv[k] + av[k] * av[k] * p * rv[k] / (rv[k] + bv[k] * bv[k] * p);
+            }
+            CHECK(std::fabs(out[ins[k].l] * out[ins[k].l] - p) <= 1e-14 * p);
+            CHECK(info[k] == 0);
+        }
+        CHECK(info[2] == 0);
+    }
     if (failures == 0) std::printf("ALL OK\n");
     return failures ? 1 : 0;
 }
