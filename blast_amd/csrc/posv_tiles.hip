@@ -7,10 +7,10 @@
 //   factor the 16x16 diagonal tile and build W = L_kk^{-1} in registers
 //   (DPP row broadcasts, Newton-refined rsqrt);
 //   L_ik = A_ik W^T on MFMA, written to HBM and back to the LDS panel;
-//   forward substitution y_kb = W b_kb, b_i -= L_ik y_kb on the RHS;
+//   forward substitution y_kb = W b_kb, b_i -= L_ik y_kb on MFMA;
 //   trailing update A_ij -= L_ik L_jk^T on MFMA for kb < j <= i.
-// Then the backward substitution x = L^{-T} y, column block by column block,
-// with L read straight from the accumulators.
+// Then the backward substitution x = L^{-T} y on MFMA, block by block, with
+// L read straight from the accumulators.
 // Algorithm reference: the potrf -> trsm<Lower> -> trsm<Upper>(trans(L))
 // sequence of dense/Potrf.hpp:66-106 and dense/Trsm.hpp:17-66; the tile
 // factorisation reorders BLAST's sums (agreement to backward error, not bits).
@@ -30,12 +30,29 @@ namespace blast_hip {
 // column, keeps W_kb = L_kb,kb^{-1} for the backward solve and holds the
 // right-hand sides.  A single wavefront needs no barriers: its LDS accesses
 // complete in program order.
+//
+// The solves run on MFMA with the right-hand sides as the B operand (columns
+// 0..kMaxRhs-1 of the 16; the others are zero):
+//   forward, at block column kb:  y_kb = W_kb r_kb,  r_i -= L_ik y_kb
+//   backward, kb = NT-1..0:       x_kb = W_kb^T (y_kb - sum_{i>kb} L_ik^T x_i)
+// Register r of an accumulator tile M (row l/16 + 4r, column l%16 of lane l)
+// is the B operand of M for K-chunk r and the A operand of M^T for the same
+// chunk, so y, x and the factor tiles feed the MFMAs without a transpose.
+// Block columns whose factor tiles are not kept in registers after the panel
+// step (the backward solve reads them back from L), so that nothing is
+// spilled: at kb = 0 the 28 trailing tiles alone fill the accumulator
+// registers.
+constexpr int kReloadCols = 2;
+
 template <int NT, bool PANEL>
 __global__ __launch_bounds__(64) void posv_wave_kernel(Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X,
                                                        int n, int nrhs, int32_t* info, const int32_t* only_if) {
-    __shared__ double sP[NT][kTileElems];          // block column kb: tile i at sP[i] (i >= kb)
-    __shared__ double sW[NT][kTileElems];          // W_kb
-    __shared__ double sR[NT * 16 * kMaxRhs + 64];  // RHS -> y -> x as [row][rhs]; tail: partial sums
+    using M = Mfma<double>;
+    constexpr int NR = NT * 16 * kMaxRhs;  // right-hand-side elements, [row][rhs]
+    constexpr int RPL = NR / 64;           // per lane
+    __shared__ double sP[NT][kTileElems];  // block column kb: tile i at sP[i] (i >= kb)
+    __shared__ double sW[NT][kTileElems];  // W_kb
+    __shared__ double sR[NR];              // b -> y -> x
 
     int64_t const b = blockIdx.x;
     if (only_if && only_if[b] >= 0) return;  // fallback mode: only the flagged instances
@@ -44,6 +61,16 @@ __global__ __launch_bounds__(64) void posv_wave_kernel(Mat<double> A, Mat<double
     bool const lpanel = L.panel != 0;
     constexpr int NTILES = NT * (NT + 1) / 2;
 
+    // right-hand sides first, then A: every load is issued before the first wait
+    double rv[RPL];
+    if (nrhs > 0) {
+        Ref<double> bb(B, b);
+#pragma unroll
+        for (int e = 0; e < RPL; ++e) {
+            int const idx = lane + 64 * e, row = idx / kMaxRhs, c = idx % kMaxRhs;
+            rv[e] = (row < n && c < nrhs) ? bb(row, c) : 0.0;
+        }
+    }
     d4 acc[NTILES];
     static_for<0, NT>([&](auto I) __attribute__((always_inline)) {
         constexpr int i = decltype(I)::value;
@@ -52,117 +79,122 @@ __global__ __launch_bounds__(64) void posv_wave_kernel(Mat<double> A, Mat<double
             acc[tri(i, j)] = load_tile<PANEL, double>(a, n, i, j, lane);
         });
     });
-    if (nrhs > 0) {
-        Ref<double> bb(B, b);
-        for (int e = lane; e < NT * 16 * kMaxRhs; e += 64) {
-            int const row = e >> 2, c = e & 3;
-            sR[e] = (row < n && c < nrhs) ? bb(row, c) : 0.0;
+    if (nrhs > 0)
+#pragma unroll
+        for (int e = 0; e < RPL; ++e) sR[lane + 64 * e] = rv[e];
+
+    // right-hand-side block blk in accumulator layout (zero in columns >= kMaxRhs)
+    int const cn = lane & 15;
+    bool const rl = cn < kMaxRhs;
+    auto rload = [&](int blk) -> d4 {
+        d4 v;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            double const t = sR[(16 * blk + M::row(lane, r)) * kMaxRhs + (cn & (kMaxRhs - 1))];
+            v[r] = rl ? t : 0.0;
         }
-    }
+        return v;
+    };
+    auto rstore = [&](int blk, d4 v) {
+        if (rl)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sR[(16 * blk + M::row(lane, r)) * kMaxRhs + cn] = v[r];
+    };
     int inf = 0;
 
     static_for<0, NT>([&](auto KB) __attribute__((always_inline)) {
         constexpr int kb = decltype(KB)::value;
         // stage block column kb
-        static_for<0, NT>([&](auto I) __attribute__((always_inline)) {
+        static_for<kb, NT>([&](auto I) __attribute__((always_inline)) {
             constexpr int i = decltype(I)::value;
-            static_for<0, i + 1>([&](auto J) __attribute__((always_inline)) {
-                constexpr int j = decltype(J)::value;
-                if constexpr (j == kb) acc_to_lds(sP[i], acc[tri(i, j)], lane);
-            });
+            acc_to_lds(sP[i], acc[tri(i, kb)], lane);
         });
         // diagonal tile: L_kk in sP[kb], W_kb in sW[kb]
         int const bad = diag_factor(sP[kb], sW[kb], lane);
         if (bad && inf == 0 && 16 * kb + bad <= n) inf = 16 * kb + bad;
         store_tile<PANEL>(l, lpanel, n, kb, kb, lds_to_acc<double>(sP[kb], lane), lane);
-        // panel: L_ik = A_ik W_kb^T
-        static_for<1, NT>([&](auto I) __attribute__((always_inline)) {
-            constexpr int i = decltype(I)::value;
-            static_for<0, i>([&](auto J) __attribute__((always_inline)) {
-                constexpr int j = decltype(J)::value;
-                if constexpr (j == kb) {
-                    d4 z = {0.0, 0.0, 0.0, 0.0};
+        // panel: L_ik = A_ik W_kb^T, two tiles' MFMA chains interleaved
+        // (more would raise the peak register count at kb = 0)
+        static_for<0, (NT - kb) / 2>([&](auto P) __attribute__((always_inline)) {
+            constexpr int i0 = kb + 1 + 2 * decltype(P)::value, i1 = i0 + 1 < NT ? i0 + 1 : i0;
+            d4 z0 = {0.0, 0.0, 0.0, 0.0}, z1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) z = Mfma<double>::mma(op_read(sP[i], q, lane), op_read(sW[kb], q, lane), z);
-                    acc[tri(i, j)] = z;
-                    store_tile<PANEL>(l, lpanel, n, i, j, z, lane);
-                    acc_to_lds(sP[i], z, lane);
-                }
-            });
+            for (int q = 0; q < 4; ++q) {
+                double const w = op_read(sW[kb], q, lane);
+                z0 = M::mma(op_read(sP[i0], q, lane), w, z0);
+                if constexpr (i1 != i0) z1 = M::mma(op_read(sP[i1], q, lane), w, z1);
+            }
+            store_tile<PANEL>(l, lpanel, n, i0, kb, z0, lane);
+            acc_to_lds(sP[i0], z0, lane);
+            if constexpr (kb >= kReloadCols) acc[tri(i0, kb)] = z0;
+            if constexpr (i1 != i0) {
+                store_tile<PANEL>(l, lpanel, n, i1, kb, z1, lane);
+                acc_to_lds(sP[i1], z1, lane);
+                if constexpr (kb >= kReloadCols) acc[tri(i1, kb)] = z1;
+            }
         });
-        // forward substitution: y_kb = W_kb b_kb; b_i -= L_ik y_kb
+        // operands L_ik (A-operand layout), shared by the forward solve and
+        // the trailing update
+        double f[NT][4];
+        static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
+            constexpr int i = decltype(I)::value;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) f[i][q] = op_read(sP[i], q, lane);
+        });
+        // forward substitution: y_kb = W_kb r_kb; r_i -= L_ik y_kb
         if (nrhs > 0) {
-            int const m = lane >> 2, c = lane & 3;
-            double rv[16];
+            d4 const rk = rload(kb);
+            d4 y = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int k = 0; k < 16; ++k) rv[k] = sR[(16 * kb + k) * kMaxRhs + c];
-            double y = 0.0;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) y = __builtin_fma(sW[kb][k * kTileStride + m], rv[k], y);
-            sR[(16 * kb + m) * kMaxRhs + c] = y;
+            for (int q = 0; q < 4; ++q) y = M::mma(op_read(sW[kb], q, lane), rk[q], y);
+            rstore(kb, y);
             static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
                 constexpr int i = decltype(I)::value;
-                double v = sR[(16 * i + m) * kMaxRhs + c];
+                d4 v = rload(i);
 #pragma unroll
-                for (int k = 0; k < 16; ++k)
-                    v = __builtin_fma(-sP[i][k * kTileStride + m], sR[(16 * kb + k) * kMaxRhs + c], v);
-                sR[(16 * i + m) * kMaxRhs + c] = v;
-                __builtin_amdgcn_sched_barrier(0);
+                for (int q = 0; q < 4; ++q) v = M::mma(-f[i][q], y[q], v);
+                rstore(i, v);
             });
         }
-        // trailing update: A_ij -= L_ik L_jk^T for kb < j <= i, operands
-        // loaded once per block column
-        double f[NT][4];
-        static_for<1, NT>([&](auto I) __attribute__((always_inline)) {
-            constexpr int i = decltype(I)::value;
-            if constexpr (i > kb)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) f[i][q] = op_read(sP[i], q, lane);
-        });
+        // trailing update: A_ij -= L_ik L_jk^T for kb < j <= i
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            static_for<1, NT>([&](auto I) __attribute__((always_inline)) {
+            static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
                 constexpr int i = decltype(I)::value;
-                static_for<1, i + 1>([&](auto J) __attribute__((always_inline)) {
+                static_for<kb + 1, i + 1>([&](auto J) __attribute__((always_inline)) {
                     constexpr int j = decltype(J)::value;
-                    if constexpr (j > kb) acc[tri(i, j)] = Mfma<double>::mma(-f[i][q], f[j][q], acc[tri(i, j)]);
+                    acc[tri(i, j)] = M::mma(-f[i][q], f[j][q], acc[tri(i, j)]);
                 });
             });
     });
 
     if (nrhs > 0) {
-        // backward substitution x = L^{-T} y with L_ik from the accumulators
-        double* sSum = sR + NT * 16 * kMaxRhs;
+        // backward substitution with L_ik^T straight from the accumulators;
+        // the first kReloadCols block columns are read back from L (this
+        // lane's own stores, so program order makes them visible), issued
+        // here so they arrive while the later columns are solved
+        static_for<0, kReloadCols>([&](auto J) __attribute__((always_inline)) {
+            constexpr int j = decltype(J)::value;
+            static_for<j + 1, NT>([&](auto I) __attribute__((always_inline)) {
+                constexpr int i = decltype(I)::value;
+                acc[tri(i, j)] = load_tile<PANEL, double>(l, n, i, j, lane);
+            });
+        });
         static_for<0, NT>([&](auto KK) __attribute__((always_inline)) {
             constexpr int kb = NT - 1 - decltype(KK)::value;
-            double p[kMaxRhs] = {0.0, 0.0, 0.0, 0.0};
+            d4 t = {0.0, 0.0, 0.0, 0.0};
             static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
                 constexpr int i = decltype(I)::value;
+                d4 const xi = rload(i);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    int const row = 16 * i + Mfma<double>::row(lane, r);
-#pragma unroll
-                    for (int c = 0; c < kMaxRhs; ++c)
-                        p[c] = __builtin_fma(acc[tri(i, kb)][r], sR[row * kMaxRhs + c], p[c]);
-                }
-                __builtin_amdgcn_sched_barrier(0);
+                for (int r = 0; r < 4; ++r) t = M::mma(acc[tri(i, kb)][r], xi[r], t);
             });
+            d4 const u = rload(kb) - t;
+            d4 x = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int c = 0; c < kMaxRhs; ++c) {
-                p[c] += __shfl_xor(p[c], 16);
-                p[c] += __shfl_xor(p[c], 32);
-            }
-            if (lane < 16)
-#pragma unroll
-                for (int c = 0; c < kMaxRhs; ++c) sSum[lane * kMaxRhs + c] = p[c];
-            int const m = lane >> 2, c = lane & 3;
-            double tv[16];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) tv[k] = sR[(16 * kb + k) * kMaxRhs + c] - sSum[k * kMaxRhs + c];
-            double x = 0.0;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) x = __builtin_fma(sW[kb][m * kTileStride + k], tv[k], x);
-            sR[(16 * kb + m) * kMaxRhs + c] = x;
+            for (int q = 0; q < 4; ++q)  // A operand W^T: W[4q + l/16][l%16]
+                x = M::mma(sW[kb][cn * kTileStride + 4 * q + (lane >> 4)], u[q], x);
+            rstore(kb, x);
         });
         Ref<double> xx(X, b);
         for (int e = lane; e < n * nrhs; e += 64) {

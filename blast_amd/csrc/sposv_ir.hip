@@ -26,6 +26,10 @@
 
 namespace blast_hip {
 
+// max that keeps a NaN operand (fmax drops it): a NaN residual, solution or
+// norm must fail the convergence test, not pass it
+__device__ __forceinline__ double max_keep_nan(double a, double b) { return (b > a || b != b) ? b : a; }
+
 template <bool PANEL>
 __device__ __forceinline__ double a_sym(Ref<double> const& a, int row, int col) {
     return row >= col ? at<PANEL>(a, row, col) : at<PANEL>(a, col, row);
@@ -203,17 +207,17 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
                         r[cc] = __builtin_fma(-av[k], sX[(c0 + k) * kMaxRhs + cc], r[cc]);
                 }
             }
-            anrm = __builtin_fmax(anrm, asum);
+            anrm = max_keep_nan(anrm, asum);
             if (row < NT * 16)
 #pragma unroll
                 for (int cc = 0; cc < kMaxRhs; ++cc) {
                     sV[row * kMaxRhs + cc] = float(r[cc]);
-                    rmax[cc] = __builtin_fmax(rmax[cc], __builtin_fabs(r[cc]));
-                    xmax[cc] = __builtin_fmax(xmax[cc], __builtin_fabs(sX[row * kMaxRhs + cc]));
+                    rmax[cc] = max_keep_nan(rmax[cc], __builtin_fabs(r[cc]));
+                    xmax[cc] = max_keep_nan(xmax[cc], __builtin_fabs(sX[row * kMaxRhs + cc]));
                 }
         }
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) anrm = __builtin_fmax(anrm, __shfl_xor(anrm, o));
+        for (int o = 32; o > 0; o >>= 1) anrm = max_keep_nan(anrm, __shfl_xor(anrm, o));
         // LAPACK dsposv: ||r|| <= ||x|| ||A|| eps sqrt(n); with tol > 0:
         // ||r|| <= tol ||x|| ||A|| (infinity norms, per right-hand side).
         double const cte = anrm * (tol > 0.0 ? tol : 1.1102230246251565e-16 * __builtin_sqrt(double(n)));
@@ -222,8 +226,8 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
         for (int cc = 0; cc < kMaxRhs; ++cc) {
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) {
-                rmax[cc] = __builtin_fmax(rmax[cc], __shfl_xor(rmax[cc], o));
-                xmax[cc] = __builtin_fmax(xmax[cc], __shfl_xor(xmax[cc], o));
+                rmax[cc] = max_keep_nan(rmax[cc], __shfl_xor(rmax[cc], o));
+                xmax[cc] = max_keep_nan(xmax[cc], __shfl_xor(xmax[cc], o));
             }
             ok = ok && (rmax[cc] <= xmax[cc] * cte);
         }

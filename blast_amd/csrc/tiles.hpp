@@ -170,71 +170,117 @@ __device__ __forceinline__ float rsqrt_nr(float d) {
     return __builtin_fmaf(r, __builtin_fmaf(-h * r, r, 0.5f), r);
 }
 
-// Factorisation of the 16x16 diagonal tile in sD, and its inverse.
-// Each 16-lane row of the wavefront holds a copy of the tile, lane i of the
-// row holding tile row i; every broadcast is a DPP row_newbcast.
-// 1. Cholesky, left-looking: for column k, lane i forms a_ik - <l_i, l_k>
-//    (row k broadcast from lane k), then r = 1/sqrt(d), l_kk = d*r,
-//    l_ik = (.)*r.
-// 2. W = L_kk^{-1}, lane i computing row i: row j is finalised at step j
-//    and broadcast from lane j to the lanes below it.
+// Sum over the four 16-lane rows of the wavefront (lane i of every row gets
+// the sum of lane i of all four rows), without LDS: v_permlane16_swap
+// exchanges odd and even rows, v_permlane32_swap the two halves.  Every lane
+// adds the same operands in the same order, so all four copies are equal.
+// The swap operands and results pass through empty asm statements: given
+// one value for both operands, the optimiser folds the two results into one
+// register (and then adds x + x).
+__device__ __forceinline__ unsigned distinct_copy(unsigned x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+template <bool ROW16>
+__device__ __forceinline__ void lane_swap(unsigned u, unsigned& mine, unsigned& other) {
+    auto const p = ROW16 ? __builtin_amdgcn_permlane16_swap(u, distinct_copy(u), false, false)
+                         : __builtin_amdgcn_permlane32_swap(u, distinct_copy(u), false, false);
+    mine = distinct_copy(p[0]);
+    other = distinct_copy(p[1]);
+}
+template <bool ROW16>
+__device__ __forceinline__ float swap_add(float x) {
+    unsigned a, b;
+    lane_swap<ROW16>(__builtin_bit_cast(unsigned, x), a, b);
+    return __builtin_bit_cast(float, a) + __builtin_bit_cast(float, b);
+}
+template <bool ROW16>
+__device__ __forceinline__ double swap_add(double x) {
+    unsigned long long const u = __builtin_bit_cast(unsigned long long, x);
+    unsigned al, bl, ah, bh;
+    lane_swap<ROW16>(unsigned(u), al, bl);
+    lane_swap<ROW16>(unsigned(u >> 32), ah, bh);
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(ah) << 32) | al) +
+           __builtin_bit_cast(double, (static_cast<unsigned long long>(bh) << 32) | bl);
+}
+template <typename T>
+__device__ __forceinline__ T rows_sum(T x) {
+    return swap_add<false>(swap_add<true>(x));
+}
+
+// Factorisation of the 16x16 diagonal tile in sD, and its inverse, by one
+// wavefront.  Lane (g, i) = (lane / 16, lane % 16) holds tile row i in the
+// columns it owns, c = 4u + g (u = 0..3): the four 16-lane rows split the
+// columns, and every broadcast of a tile row is a DPP row_newbcast.
+// 1. Cholesky, left-looking: for column k, each row-group forms its share of
+//    a_ik - <l_i, l_k> over its own columns j < k (l_kj broadcast from lane
+//    k), the shares are summed across the groups (rows_sum), then
+//    r = 1/sqrt(d), l_kk = d*r, l_ik = (.)*r, kept by the owner of column k.
+// 2. W = L_kk^{-1}, lane (g, i) computing W[i][c] for its columns: row j is
+//    finalised at step j and broadcast from lane j to the lanes below it;
+//    the multipliers l_ij come from the factor written back to sD.
 // On exit sD holds L_kk (zeros above the diagonal) and sW holds W, both as
 // T[m][k] at k*17 + m.  Returns 0 or the 1-based tile column of the first
-// non-positive pivot.  Branch-free: every lane runs the same instructions.
+// non-positive pivot (the same value in every lane).  Branch-free: every
+// lane runs the same instructions; all 64 lanes must be active.
 template <typename T>
 __device__ int diag_factor(T* sD, T* sW, int lane) {
-    int const i = lane & 15;
-    T t[16];
+    int const i = lane & 15, g = lane >> 4;
+    T t[4];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) t[c] = sD[c * kTileStride + i];
+    for (int u = 0; u < 4; ++u) t[u] = sD[(4 * u + g) * kTileStride + i];
     int bad = 0;
     T myr = T(0);
     static_for<0, 16>([&](auto K) __attribute__((always_inline)) {
-        constexpr int k = decltype(K)::value;
-        T acc0 = t[k], acc1 = T(0);
-        static_for<0, k>([&](auto J) __attribute__((always_inline)) {
-            constexpr int j = decltype(J)::value;
-            if constexpr (j % 2 == 0)
-                acc0 = __builtin_fma(-t[j], row_bcast<k>(t[j]), acc0);
-            else
-                acc1 = __builtin_fma(-t[j], row_bcast<k>(t[j]), acc1);
+        constexpr int k = decltype(K)::value, gk = k & 3, uk = k >> 2;
+        T p0 = (g == gk) ? t[uk] : T(0), p1 = T(0);
+        static_for<0, 4>([&](auto U) __attribute__((always_inline)) {
+            constexpr int u = decltype(U)::value;
+            if constexpr (4 * u < k) {
+                // column 4u + g: finished (j < k) in every group when 4u + 3 < k
+                T const lij = (4 * u + 3 < k || 4 * u + g < k) ? t[u] : T(0);
+                T const lkj = row_bcast<k>(t[u]);
+                if constexpr (u % 2 == 0)
+                    p0 = __builtin_fma(-lij, lkj, p0);
+                else
+                    p1 = __builtin_fma(-lij, lkj, p1);
+            }
         });
-        T const v = acc0 + acc1;
-        T const d = row_bcast<k>(v);
+        T const v = rows_sum(k > 4 ? p0 + p1 : p0);
+        T const d = row_bcast<k>(v);  // equals v in lane k
         bad = (bad == 0 && !(d > T(0))) ? k + 1 : bad;
         T const r = rsqrt_nr(d);
         myr = (i == k) ? r : myr;
-        T lik = ((i == k) ? d : v) * r;
-        lik = (i < k) ? T(0) : lik;
-        t[k] = lik;
+        // rows i < k hold values that are never used (the final write masks
+        // them; rows_sum and the broadcasts of rows >= k never read them)
+        t[uk] = (g == gk) ? v * r : t[uk];
         __builtin_amdgcn_sched_barrier(0);
     });
-    // W row by row: at step j lane j finalises W[j][:] = (e_j - acc) / l_jj;
-    // lanes i > j then accumulate acc[c] += l_ij W[j][c], c <= j.
-    T w[16];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) w[c] = T(0);
+    for (int u = 0; u < 4; ++u) sD[(4 * u + g) * kTileStride + i] = (4 * u + g <= i) ? t[u] : T(0);
+    // W row by row: at step j lane j finalises W[j][c] = (delta_jc - acc) / l_jj
+    // for its columns c <= j; lanes i > j then accumulate acc += l_ij W[j][c].
+    // Lane j finalises with one FMA, w = w*s + e: s = -1/l_jj (s = 1 in the
+    // other lanes), e = 1/l_jj in the column c == j; w is 0 for c > j.
+    T w[4] = {T(0), T(0), T(0), T(0)};
     static_for<0, 16>([&](auto J) __attribute__((always_inline)) {
         constexpr int j = decltype(J)::value;
-        static_for<0, j>([&](auto C) __attribute__((always_inline)) {
-            constexpr int c = decltype(C)::value;
-            w[c] = (i == j) ? -w[c] * myr : w[c];
-        });
-        w[j] = (i == j) ? myr : w[j];
-        T const f = (i > j) ? t[j] : T(0);
-        static_for<0, j + 1>([&](auto C) __attribute__((always_inline)) {
-            constexpr int c = decltype(C)::value;
-            w[c] = __builtin_fma(f, row_bcast<j>(w[c]), w[c]);
+        T const f = (i > j) ? sD[j * kTileStride + i] : T(0);  // l_ij
+        T const sj = (i == j) ? -myr : T(1);
+        static_for<0, 4>([&](auto U) __attribute__((always_inline)) {
+            constexpr int u = decltype(U)::value;
+            if constexpr (4 * u <= j) {
+                if constexpr (j - 4 * u < 4)
+                    w[u] = __builtin_fma(w[u], sj, (i == j && g == j - 4 * u) ? myr : T(0));
+                else
+                    w[u] = w[u] * sj;
+                w[u] = __builtin_fma(f, row_bcast<j>(w[u]), w[u]);
+            }
         });
         __builtin_amdgcn_sched_barrier(0);
     });
-    if (lane < 16) {
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
-            sD[c * kTileStride + i] = (c <= i) ? t[c] : T(0);  // L[i][c]
-            sW[c * kTileStride + i] = (c <= i) ? w[c] : T(0);  // W[i][c]
-        }
-    }
+    for (int u = 0; u < 4; ++u) sW[(4 * u + g) * kTileStride + i] = (4 * u + g <= i) ? w[u] : T(0);
     return bad;
 }
 
