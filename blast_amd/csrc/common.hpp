@@ -47,6 +47,11 @@ struct Mat {
     __device__ __forceinline__ T* inst(int64_t b) const { return data + b * stride; }
 };
 
+// Fused multiply-add in the operand type: __builtin_fma is the double
+// routine, so float operands would be widened and the result rounded twice.
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // Per-instance accessor bound to one instance's base pointer.
 template <typename T>
 struct Ref {

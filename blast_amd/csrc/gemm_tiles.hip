@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(T alpha, Mat<T> A, Mat<T
                         d(i, j) = (MODE == kTrmmLeftUpper || MODE == kTrmmRightLower || MODE == kUpperTimesLower ||
                                    MODE == kGemmOverwrite)
                                       ? ab
-                                                                                       : __builtin_fma(beta, c(i, j), ab);
+                                                                                       : fma_t(beta, c(i, j), ab);
                     }
                 }
         }

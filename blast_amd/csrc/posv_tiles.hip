@@ -44,7 +44,9 @@ namespace blast_hip {
 // registers.
 constexpr int kReloadCols = 2;
 
-template <int NT, bool PANEL>
+// FULL: n == 16 NT and panel storage, tiles loaded without bounds tests
+// (load_tile_full).
+template <int NT, bool PANEL, bool FULL>
 __global__ __launch_bounds__(64) void posv_wave_kernel(Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X,
                                                        int n, int nrhs, int32_t* info, const int32_t* only_if) {
     using M = Mfma<double>;
@@ -76,7 +78,7 @@ __global__ __launch_bounds__(64) void posv_wave_kernel(Mat<double> A, Mat<double
         constexpr int i = decltype(I)::value;
         static_for<0, i + 1>([&](auto J) __attribute__((always_inline)) {
             constexpr int j = decltype(J)::value;
-            acc[tri(i, j)] = load_tile<PANEL, double>(a, n, i, j, lane);
+            acc[tri(i, j)] = FULL ? load_tile_full<PANEL, double>(a, i, j, lane) : load_tile<PANEL, double>(a, n, i, j, lane);
         });
     });
     if (nrhs > 0)
@@ -177,7 +179,7 @@ __global__ __launch_bounds__(64) void posv_wave_kernel(Mat<double> A, Mat<double
             constexpr int j = decltype(J)::value;
             static_for<j + 1, NT>([&](auto I) __attribute__((always_inline)) {
                 constexpr int i = decltype(I)::value;
-                acc[tri(i, j)] = load_tile<PANEL, double>(l, n, i, j, lane);
+                acc[tri(i, j)] = FULL ? load_tile_full<PANEL, double>(l, i, j, lane) : load_tile<PANEL, double>(l, n, i, j, lane);
             });
         });
         static_for<0, NT>([&](auto KK) __attribute__((always_inline)) {
@@ -205,16 +207,25 @@ __global__ __launch_bounds__(64) void posv_wave_kernel(Mat<double> A, Mat<double
     if (info && lane == 0) info[b] = inf;
 }
 
+template <int NT, bool PANEL>
+static void launch_nt_full(hipStream_t s, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int n, int nrhs,
+                           int64_t batch, int32_t* info, const int32_t* only_if) {
+    if (PANEL && n == 16 * NT)  // (dense storage: the generic addressing would spill)
+        hipLaunchKernelGGL((posv_wave_kernel<NT, PANEL, PANEL>), dim3(unsigned(batch)), dim3(64), 0, s, A, L, B, X, n,
+                           nrhs, info, only_if);
+    else
+        hipLaunchKernelGGL((posv_wave_kernel<NT, PANEL, false>), dim3(unsigned(batch)), dim3(64), 0, s, A, L, B, X, n,
+                           nrhs, info, only_if);
+}
+
 template <int NT>
 static int launch_nt(hipStream_t s, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int n, int nrhs,
                      int64_t batch, int32_t* info, const int32_t* only_if) {
     bool const panel = A.panel && !A.trans && L.panel && !L.trans;
     if (panel)
-        hipLaunchKernelGGL((posv_wave_kernel<NT, true>), dim3(unsigned(batch)), dim3(64), 0, s, A, L, B, X, n, nrhs,
-                           info, only_if);
+        launch_nt_full<NT, true>(s, A, L, B, X, n, nrhs, batch, info, only_if);
     else
-        hipLaunchKernelGGL((posv_wave_kernel<NT, false>), dim3(unsigned(batch)), dim3(64), 0, s, A, L, B, X, n,
-                           nrhs, info, only_if);
+        launch_nt_full<NT, false>(s, A, L, B, X, n, nrhs, batch, info, only_if);
     return check_launch("posv_wave_kernel");
 }
 

@@ -35,9 +35,9 @@ __global__ __launch_bounds__(kThreads) void gemm_ref_kernel(T alpha, Mat<T> A, M
     for (int e = threadIdx.x; e < M * N; e += kThreads) {
         int const i = e % M, j = e / M;
         T acc = T(0);
-        for (int k = 0; k < K; ++k) acc = __builtin_fma(a(i, k), bb(k, j), acc);
+        for (int k = 0; k < K; ++k) acc = fma_t(a(i, k), bb(k, j), acc);
         acc = acc * alpha;
-        d(i, j) = __builtin_fma(beta, c(i, j), acc);
+        d(i, j) = fma_t(beta, c(i, j), acc);
     }
 }
 
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kThreads) void syrk_ref_kernel(T alpha, Mat<T> A, T
         int const i = e % M, j = e / M;
         if (i < j) continue;
         T acc = beta * c(i, j);
-        for (int k = 0; k < K; ++k) acc = __builtin_fma(alpha * a(i, k), a(j, k), acc);
+        for (int k = 0; k < K; ++k) acc = fma_t(alpha * a(i, k), a(j, k), acc);
         d(i, j) = acc;
     }
 }
@@ -65,9 +65,9 @@ __global__ __launch_bounds__(kThreads) void trmm_ref_kernel(int side_right, T al
         int const i = e % M, j = e / M;
         T acc = T(0);
         if (!side_right)
-            for (int k = i; k < M; ++k) acc = __builtin_fma(alpha * a(i, k), bb(k, j), acc);
+            for (int k = i; k < M; ++k) acc = fma_t(alpha * a(i, k), bb(k, j), acc);
         else
-            for (int k = j; k < N; ++k) acc = __builtin_fma(alpha * bb(i, k), a(k, j), acc);
+            for (int k = j; k < N; ++k) acc = fma_t(alpha * bb(i, k), a(k, j), acc);
         c(i, j) = acc;
     }
 }
@@ -91,14 +91,14 @@ __global__ __launch_bounds__(kThreads) void trsm_ref_kernel(int side_right, int 
             for (int j = threadIdx.x; j < N; j += kThreads)
                 for (int i = M - 1; i >= 0; --i) {
                     T v = alpha * bb(i, j);
-                    for (int k = i + 1; k < M; ++k) v = __builtin_fma(-a(i, k), x(k, j), v);
+                    for (int k = i + 1; k < M; ++k) v = fma_t(-a(i, k), x(k, j), v);
                     x(i, j) = unit ? v : v / a(i, i);
                 }
         } else {
             for (int i = threadIdx.x; i < M; i += kThreads)
                 for (int j = N - 1; j >= 0; --j) {
                     T v = alpha * bb(i, j);
-                    for (int k = j + 1; k < N; ++k) v = __builtin_fma(-x(i, k), a(k, j), v);
+                    for (int k = j + 1; k < N; ++k) v = fma_t(-x(i, k), a(k, j), v);
                     x(i, j) = unit ? v : v / a(j, j);
                 }
         }
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kThreads) void trsm_ref_kernel(int side_right, int 
             int const nr = M - k - 1;
             for (int e = threadIdx.x; e < nr * N; e += kThreads) {
                 int const i = k + 1 + e % nr, j = e / nr;
-                x(i, j) = __builtin_fma(-a(i, k), x(k, j), x(i, j));
+                x(i, j) = fma_t(-a(i, k), x(k, j), x(i, j));
             }
             __syncthreads();
         }
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kThreads) void trsm_ref_kernel(int side_right, int 
             int const nc = N - k - 1;
             for (int e = threadIdx.x; e < M * nc; e += kThreads) {
                 int const i = e % M, j = k + 1 + e / M;
-                x(i, j) = __builtin_fma(-x(i, k), a(k, j), x(i, j));
+                x(i, j) = fma_t(-x(i, k), a(k, j), x(i, j));
             }
             __syncthreads();
         }
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kThreads) void potrf_ref_kernel(Mat<T> A, Mat<T> L,
             int const nr = M - k - 1, nc = N - k - 1;
             for (int e = threadIdx.x; e < nr * nc; e += kThreads) {
                 int const i = k + 1 + e % nr, j = k + 1 + e / nr;
-                if (i >= j) l(i, j) = __builtin_fma(-l(i, k), l(j, k), l(i, j));
+                if (i >= j) l(i, j) = fma_t(-l(i, k), l(j, k), l(i, j));
             }
             __syncthreads();
         }
@@ -229,7 +229,7 @@ __device__ int getf2_block(Ref<T> const& a, int r0, int c0, int m, int n, int64_
         int const nr = m - k - 1, nc = n - k - 1;
         for (int e = threadIdx.x; e < nr * nc; e += kThreads) {
             int const i = k + 1 + e % nr, j = k + 1 + e / nr;
-            a(r0 + i, c0 + j) = __builtin_fma(-a(r0 + i, c0 + k), a(r0 + k, c0 + j), a(r0 + i, c0 + j));
+            a(r0 + i, c0 + j) = fma_t(-a(r0 + i, c0 + k), a(r0 + k, c0 + j), a(r0 + i, c0 + j));
         }
         __syncthreads();
     }
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(kThreads) void getrf_ref_kernel(Mat<T> A, int64_t* 
             int const nr = M - k - 1, nc = N - k - 1;
             for (int e = threadIdx.x; e < nr * nc; e += kThreads) {
                 int const i = k + 1 + e % nr, j = k + 1 + e / nr;
-                a(i, j) = __builtin_fma(-a(i, k), a(k, j), a(i, j));
+                a(i, j) = fma_t(-a(i, k), a(k, j), a(i, j));
             }
             if (threadIdx.x == 0) piv[k] = k;
             __syncthreads();
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(kThreads) void getrf_ref_kernel(Mat<T> A, int64_t* 
         for (int j = k + NB + threadIdx.x; j < N; j += kThreads)
             for (int i = 0; i < NB; ++i) {
                 T v = a(k + i, j);
-                for (int kk = 0; kk < i; ++kk) v = __builtin_fma(-a(k + i, k + kk), a(k + kk, j), v);
+                for (int kk = 0; kk < i; ++kk) v = fma_t(-a(k + i, k + kk), a(k + kk, j), v);
                 a(k + i, j) = v;
             }
         __syncthreads();
@@ -299,9 +299,9 @@ __global__ __launch_bounds__(kThreads) void getrf_ref_kernel(Mat<T> A, int64_t* 
         for (int e = threadIdx.x; e < nr * nc; e += kThreads) {
             int const i = k + NB + e % nr, j = k + NB + e / nr;
             T acc = T(0);
-            for (int kk = 0; kk < NB; ++kk) acc = __builtin_fma(a(i, k + kk), a(k + kk, j), acc);
+            for (int kk = 0; kk < NB; ++kk) acc = fma_t(a(i, k + kk), a(k + kk, j), acc);
             acc = acc * T(-1);
-            a(i, j) = __builtin_fma(T(1), a(i, j), acc);
+            a(i, j) = fma_t(T(1), a(i, j), acc);
         }
         __syncthreads();
     }

@@ -35,7 +35,9 @@ __device__ __forceinline__ double a_sym(Ref<double> const& a, int row, int col) 
     return row >= col ? at<PANEL>(a, row, col) : at<PANEL>(a, col, row);
 }
 
-template <int NT, bool PANEL, bool BF16>
+// FULL: n == 16 NT and panel storage, tiles loaded without bounds tests
+// (load_tile_full).
+template <int NT, bool PANEL, bool BF16, bool FULL>
 __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<double> B, Mat<double> X, int n,
                                                          int nrhs, int itmax, double tol, int32_t* info,
                                                          int32_t* iter) {
@@ -61,7 +63,7 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
         constexpr int i = decltype(I)::value;
         static_for<0, i + 1>([&](auto J) __attribute__((always_inline)) {
             constexpr int j = decltype(J)::value;
-            acc[tri(i, j)] = load_tile<PANEL, float>(a, n, i, j, lane);
+            acc[tri(i, j)] = FULL ? load_tile_full<PANEL, float>(a, i, j, lane) : load_tile<PANEL, float>(a, n, i, j, lane);
         });
     });
     static_for<0, NT>([&](auto KB) __attribute__((always_inline)) {
@@ -263,15 +265,24 @@ int launch_fill_i32(hipStream_t s, int32_t* p, int32_t v, int64_t count) {
     return check_launch("fill_i32_kernel");
 }
 
+template <int NT, bool PANEL, bool BF16>
+static void launch_ir_full(hipStream_t s, Mat<double> A, Mat<double> B, Mat<double> X, int n, int nrhs, int itmax,
+                           double tol, int64_t batch, int32_t* info, int32_t* iter) {
+    if (PANEL && n == 16 * NT)  // (dense storage: the generic addressing would spill)
+        hipLaunchKernelGGL((sposv_ir_kernel<NT, PANEL, BF16, PANEL>), dim3(unsigned(batch)), dim3(64), 0, s, A, B, X, n,
+                           nrhs, itmax, tol, info, iter);
+    else
+        hipLaunchKernelGGL((sposv_ir_kernel<NT, PANEL, BF16, false>), dim3(unsigned(batch)), dim3(64), 0, s, A, B, X,
+                           n, nrhs, itmax, tol, info, iter);
+}
+
 template <int NT, bool BF16>
 static int launch_ir(hipStream_t s, Mat<double> A, Mat<double> B, Mat<double> X, int n, int nrhs, int itmax,
                      double tol, int64_t batch, int32_t* info, int32_t* iter) {
     if (A.panel && !A.trans)
-        hipLaunchKernelGGL((sposv_ir_kernel<NT, true, BF16>), dim3(unsigned(batch)), dim3(64), 0, s, A, B, X, n, nrhs,
-                           itmax, tol, info, iter);
+        launch_ir_full<NT, true, BF16>(s, A, B, X, n, nrhs, itmax, tol, batch, info, iter);
     else
-        hipLaunchKernelGGL((sposv_ir_kernel<NT, false, BF16>), dim3(unsigned(batch)), dim3(64), 0, s, A, B, X, n,
-                           nrhs, itmax, tol, info, iter);
+        launch_ir_full<NT, false, BF16>(s, A, B, X, n, nrhs, itmax, tol, batch, info, iter);
     return check_launch("sposv_ir_kernel");
 }
 

@@ -130,6 +130,27 @@ __device__ __forceinline__ typename Mfma<T>::acc_t load_tile(Ref<double> const& 
     return v;
 }
 
+// Tile (ti, tj) of A when n is a multiple of 16 (every tile inside the
+// matrix): no bounds to test, so the loads carry no branches and all of them
+// issue before the first use.  Every address is a lower-triangle element
+// that load_tile also reads: the strict upper part of a diagonal tile is
+// read from its mirror (a lower element) and then zeroed.
+template <bool PANEL, typename T>
+__device__ __forceinline__ typename Mfma<T>::acc_t load_tile_full(Ref<double> const& a, int ti, int tj, int lane) {
+    typename Mfma<T>::acc_t v;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        int const row = 16 * ti + Mfma<T>::row(lane, r), col = 16 * tj + acc_col(lane);
+        if (ti == tj) {
+            double const x = at<PANEL>(a, row >= col ? row : col, row >= col ? col : row);
+            v[r] = T(row >= col ? x : 0.0);
+        } else {
+            v[r] = T(at<PANEL>(a, row, col));
+        }
+    }
+    return v;
+}
+
 // Store FP64 tile (ti, tj) of L.  Dense storage: i >= j only.  Panel storage:
 // also zeros in the strict upper part of each 4x4 diagonal block (what
 // BLAST's panel potrf leaves there is rounding residue of 0).
@@ -160,8 +181,8 @@ __device__ __forceinline__ float row_bcast(float v) {
 __device__ __forceinline__ double rsqrt_nr(double d) {
     double r = __builtin_amdgcn_rsq(d);
     double const h = 0.5 * d;
-    r = __builtin_fma(r, __builtin_fma(-h * r, r, 0.5), r);
-    r = __builtin_fma(r, __builtin_fma(-h * r, r, 0.5), r);
+    r = fma_t(r, fma_t(-h * r, r, 0.5), r);
+    r = fma_t(r, fma_t(-h * r, r, 0.5), r);
     return r;
 }
 __device__ __forceinline__ float rsqrt_nr(float d) {
@@ -241,9 +262,9 @@ __device__ int diag_factor(T* sD, T* sW, int lane) {
                 T const lij = (4 * u + 3 < k || 4 * u + g < k) ? t[u] : T(0);
                 T const lkj = row_bcast<k>(t[u]);
                 if constexpr (u % 2 == 0)
-                    p0 = __builtin_fma(-lij, lkj, p0);
+                    p0 = fma_t(-lij, lkj, p0);
                 else
-                    p1 = __builtin_fma(-lij, lkj, p1);
+                    p1 = fma_t(-lij, lkj, p1);
             }
         });
         T const v = rows_sum(k > 4 ? p0 + p1 : p0);
@@ -271,10 +292,10 @@ __device__ int diag_factor(T* sD, T* sW, int lane) {
             constexpr int u = decltype(U)::value;
             if constexpr (4 * u <= j) {
                 if constexpr (j - 4 * u < 4)
-                    w[u] = __builtin_fma(w[u], sj, (i == j && g == j - 4 * u) ? myr : T(0));
+                    w[u] = fma_t(w[u], sj, (i == j && g == j - 4 * u) ? myr : T(0));
                 else
                     w[u] = w[u] * sj;
-                w[u] = __builtin_fma(f, row_bcast<j>(w[u]), w[u]);
+                w[u] = fma_t(f, row_bcast<j>(w[u]), w[u]);
             }
         });
         __builtin_amdgcn_sched_barrier(0);
@@ -313,7 +334,7 @@ __device__ void tri_inverse(const T* sA, T* sW, int lane, bool upper, bool unit)
         T const f = (i > j) ? t[j] : T(0);
         static_for<0, j + 1>([&](auto C) __attribute__((always_inline)) {
             constexpr int c = decltype(C)::value;
-            w[c] = __builtin_fma(f, row_bcast<j>(w[c]), w[c]);
+            w[c] = fma_t(f, row_bcast<j>(w[c]), w[c]);
         });
         __builtin_amdgcn_sched_barrier(0);
     });

@@ -235,7 +235,7 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
                         row[c * kTileStride] = l;
                         for (int cc = c + 1; cc < 16; ++cc)
                             row[cc * kTileStride] =
-                                __builtin_fma(-l, pp[c >> 4][cc * kTileStride + (c & 15)], row[cc * kTileStride]);
+                                fma_t(-l, pp[c >> 4][cc * kTileStride + (c & 15)], row[cc * kTileStride]);
                     }
                 }
                 __syncthreads();
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
                     for (int i = 0; i < 16; ++i) {
                         T s = T(0);
 #pragma unroll
-                        for (int k = 0; k < i; ++k) s = __builtin_fma(-d[k * kTileStride + i], x[k], s);
+                        for (int k = 0; k < i; ++k) s = fma_t(-d[k * kTileStride + i], x[k], s);
                         x[i] = (i == cc) ? T(1) : (i < cc ? T(0) : s);
                     }
 #pragma unroll
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
                     for (int i = 15; i >= 0; --i) {
                         T s = T(0);
 #pragma unroll
-                        for (int k = i + 1; k < 16; ++k) s = __builtin_fma(-d[k * kTileStride + i], x[k], s);
+                        for (int k = i + 1; k < 16; ++k) s = fma_t(-d[k * kTileStride + i], x[k], s);
                         T const u = d[i * kTileStride + i];
                         x[i] = (i > cc) ? T(0) : ((i == cc ? T(1) : s) / u);
                     }
