@@ -43,7 +43,7 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
                                                          int32_t* iter) {
     constexpr int NTILES = NT * (NT + 1) / 2;
     constexpr int NV = NT * 16 * kMaxRhs;
-    __shared__ float sP[NT][kTileElems];  // block column kb (operand layout); sP[kb]: the diagonal tile
+    __shared__ __align__(16) float sP[NT][kTileElems];  // block column kb (operand layout); sP[kb]: the diagonal tile
     __shared__ float sW[kTileElems];      // W_kb while factoring; one staged tile in the solves
     __shared__ double sX[NV];             // x, [row][rhs]
     __shared__ float sV[NV];              // FP32 solve vector, [row][rhs]
@@ -181,42 +181,87 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
     for (;; ++it) {
         solve();  // d = L^{-T} L^{-1} v
         for (int e = lane; e < NV; e += 64) sX[e] += double(sV[e]);
-        // r = b - A x in FP64, A (symmetric, lower triangle stored) read in
-        // 16-column batches so the loads overlap; v = float(r)
         double rmax[kMaxRhs] = {0.0, 0.0, 0.0, 0.0}, xmax[kMaxRhs] = {0.0, 0.0, 0.0, 0.0};
-        for (int h = 0; h < NT * 16; h += 64) {
-            int const row = h + lane;
-            double r[kMaxRhs];
+        if constexpr (FULL) {
+            // r = b - A x in FP64, lane = row, A symmetric from its lower
+            // triangle in panel storage: element (row, col) at
+            // (row/4)*spacing + row%4 + 4 col for col <= row, its mirror
+            // (col/4)*spacing + col%4 + 4 row above.  The column is the same
+            // in every lane, so the mirror's panel offset is scalar
+            // arithmetic; consecutive columns reuse each lane's cache lines.
+            // x is broadcast from LDS.
+            for (int h = 0; h < NT * 16; h += 64) {
+                int const row = h + lane;
+                bool const act = row < NT * 16;
+                int const rr = act ? row : NT * 16 - 1;  // idle lanes read a valid row
+                int64_t const lowbase = int64_t(rr >> 2) * A.spacing + (rr & 3);
+                double r[kMaxRhs];
 #pragma unroll
-            for (int cc = 0; cc < kMaxRhs; ++cc) r[cc] = (row < n && cc < nrhs) ? bb(row, cc) : 0.0;
-            double asum = 0.0;
-            for (int c0 = 0; c0 < NT * 16; c0 += 16) {
-                double av[16];
+                for (int cc = 0; cc < kMaxRhs; ++cc) r[cc] = cc < nrhs ? bb(rr, cc) : 0.0;
+                double asum = 0.0;
+                for (int c0 = 0; c0 < NT * 16; c0 += 16) {
+                    double av[16];
 #pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    int const col = c0 + k;
-                    bool const in = row < n && col < n;
-                    int const lo = row >= col ? row : col, hi = row >= col ? col : row;
-                    int64_t const off = in ? (PANEL ? int64_t(lo >> 2) * A.spacing + (lo & 3) + 4 * hi : A.off(lo, hi)) : 0;
-                    double const v = a.p[off];
-                    av[k] = in ? v : 0.0;
+                    for (int k = 0; k < 16; ++k) {
+                        int const col = c0 + k;
+                        int64_t const up = int64_t(col >> 2) * A.spacing + (col & 3);  // scalar
+                        av[k] = a.p[rr >= col ? lowbase + 4 * col : up + 4 * rr];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) {
+                        asum += __builtin_fabs(av[k]);
+#pragma unroll
+                        for (int cc = 0; cc < kMaxRhs; ++cc)
+                            r[cc] = fma_t(-av[k], sX[(c0 + k) * kMaxRhs + cc], r[cc]);
+                    }
                 }
+                if (act) {
+                    anrm = max_keep_nan(anrm, asum);
 #pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    asum += __builtin_fabs(av[k]);
-#pragma unroll
-                    for (int cc = 0; cc < kMaxRhs; ++cc)
-                        r[cc] = __builtin_fma(-av[k], sX[(c0 + k) * kMaxRhs + cc], r[cc]);
+                    for (int cc = 0; cc < kMaxRhs; ++cc) {
+                        sV[row * kMaxRhs + cc] = float(r[cc]);
+                        rmax[cc] = max_keep_nan(rmax[cc], __builtin_fabs(r[cc]));
+                        xmax[cc] = max_keep_nan(xmax[cc], __builtin_fabs(sX[row * kMaxRhs + cc]));
+                    }
                 }
             }
-            anrm = max_keep_nan(anrm, asum);
-            if (row < NT * 16)
+        } else {
+            // r = b - A x in FP64, A (symmetric, lower triangle stored) read in
+            // 16-column batches so the loads overlap; v = float(r)
+            for (int h = 0; h < NT * 16; h += 64) {
+                int const row = h + lane;
+                double r[kMaxRhs];
 #pragma unroll
-                for (int cc = 0; cc < kMaxRhs; ++cc) {
-                    sV[row * kMaxRhs + cc] = float(r[cc]);
-                    rmax[cc] = max_keep_nan(rmax[cc], __builtin_fabs(r[cc]));
-                    xmax[cc] = max_keep_nan(xmax[cc], __builtin_fabs(sX[row * kMaxRhs + cc]));
+                for (int cc = 0; cc < kMaxRhs; ++cc) r[cc] = (row < n && cc < nrhs) ? bb(row, cc) : 0.0;
+                double asum = 0.0;
+                for (int c0 = 0; c0 < NT * 16; c0 += 16) {
+                    double av[16];
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) {
+                        int const col = c0 + k;
+                        bool const in = row < n && col < n;
+                        int const lo = row >= col ? row : col, hi = row >= col ? col : row;
+                        int64_t const off = in ? (PANEL ? int64_t(lo >> 2) * A.spacing + (lo & 3) + 4 * hi : A.off(lo, hi)) : 0;
+                        double const v = a.p[off];
+                        av[k] = in ? v : 0.0;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) {
+                        asum += __builtin_fabs(av[k]);
+#pragma unroll
+                        for (int cc = 0; cc < kMaxRhs; ++cc)
+                            r[cc] = __builtin_fma(-av[k], sX[(c0 + k) * kMaxRhs + cc], r[cc]);
+                    }
                 }
+                anrm = max_keep_nan(anrm, asum);
+                if (row < NT * 16)
+#pragma unroll
+                    for (int cc = 0; cc < kMaxRhs; ++cc) {
+                        sV[row * kMaxRhs + cc] = float(r[cc]);
+                        rmax[cc] = max_keep_nan(rmax[cc], __builtin_fabs(r[cc]));
+                        xmax[cc] = max_keep_nan(xmax[cc], __builtin_fabs(sX[row * kMaxRhs + cc]));
+                    }
+            }
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) anrm = max_keep_nan(anrm, __shfl_xor(anrm, o));
