@@ -7,8 +7,8 @@
 //   1. factor A (rounded to FP32) with the right-looking tile algorithm of
 //      posv_tiles.hip on v_mfma_f32_16x16x4f32 (or, with BF16, trailing
 //      updates on v_mfma_f32_16x16x16bf16_1k: BF16 products, FP32
-//      accumulation); the factor tiles stay in the accumulator registers,
-//      each diagonal tile replaced by W = L_kk^{-1};
+//      accumulation); the off-diagonal factor tiles stay in the accumulator
+//      registers, the diagonal ones are replaced by W = L_kk^{-1} in LDS;
 //   2. x = 0, r = b; repeat: d = L^{-T} L^{-1} r in FP32 on MFMA, x += d in FP64,
 //      r = b - A x in FP64 with A read again from HBM (L2/MALL resident),
 //      until max|r| <= max|x| * ||A||_inf * eps * sqrt(n) per right-hand side
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
     constexpr int NTILES = NT * (NT + 1) / 2;
     constexpr int NV = NT * 16 * kMaxRhs;
     __shared__ __align__(16) float sP[NT][kTileElems];  // block column kb (operand layout); sP[kb]: the diagonal tile
-    __shared__ float sW[kTileElems];      // W_kb while factoring; one staged tile in the solves
+    __shared__ float sW[kTileElems];      // one staged factor tile in the forward solve
     __shared__ double sX[NV];             // x, [row][rhs]
     __shared__ float sV[NV];              // FP32 solve vector, [row][rhs]
 
@@ -55,8 +55,8 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
     using acc_t = M::acc_t;
 
     // ---------------------------------------------------------- factor
-    // acc holds the FP32 factor for the whole kernel: L_ij below the
-    // diagonal and W_k = L_kk^{-1} on it.
+    // acc holds the FP32 factor tiles L_ij below the diagonal for the whole
+    // kernel; sP[k] ends up holding W_k = L_kk^{-1}.
     acc_t acc[NTILES];
     int bad = 0;
     static_for<0, NT>([&](auto I) __attribute__((always_inline)) {
@@ -72,14 +72,15 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
             constexpr int i = decltype(I)::value;
             acc_to_lds<float>(sP[i], acc[tri(i, kb)], lane);
         });
-        int const bk = diag_factor<float>(sP[kb], sW, lane);
+        // W_kb = L_kk^{-1} replaces the diagonal tile in sP[kb] (the factor
+        // L_kk itself is not needed); sP[kb] is not written again
+        int const bk = diag_factor<float>(sP[kb], sP[kb], lane);
         if (bk && bad == 0 && 16 * kb + bk <= n) bad = 16 * kb + bk;
-        acc[tri(kb, kb)] = lds_to_acc<float>(sW, lane);
         static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
             constexpr int i = decltype(I)::value;
             acc_t z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) z = M::mma(op_read(sP[i], q, lane), op_read(sW, q, lane), z);
+            for (int q = 0; q < 4; ++q) z = M::mma(op_read(sP[i], q, lane), op_read(sP[kb], q, lane), z);
             acc[tri(i, kb)] = z;
             acc_to_lds<float>(sP[i], z, lane);
         });
@@ -138,10 +139,9 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
     auto solve = [&]() {
         static_for<0, NT>([&](auto KB) __attribute__((always_inline)) {
             constexpr int kb = decltype(KB)::value;
-            acc_to_lds<float>(sW, acc[tri(kb, kb)], lane);
             acc_t y = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) y = M::mma(op_read(sW, q, lane), vb(kb, 4 * q + (lane >> 4)), y);
+            for (int q = 0; q < 4; ++q) y = M::mma(op_read(sP[kb], q, lane), vb(kb, 4 * q + (lane >> 4)), y);
             vstore(kb, y);
             static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
                 constexpr int i = decltype(I)::value;
@@ -156,7 +156,8 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
             constexpr int kb = NT - 1 - decltype(KK)::value;
             acc_t x = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) x = M::mma(acc[tri(kb, kb)][q], vb(kb, M::row(lane, q)), x);
+            for (int q = 0; q < 4; ++q)  // A operand W^T: W[row(l, q)][l % 16]
+                x = M::mma(sP[kb][cn * kTileStride + M::row(lane, q)], vb(kb, M::row(lane, q)), x);
             vstore(kb, x);
             static_for<0, kb>([&](auto J) __attribute__((always_inline)) {
                 constexpr int j = decltype(J)::value;
