@@ -37,6 +37,8 @@ __device__ __forceinline__ double a_sym(Ref<double> const& a, int row, int col) 
 
 // FULL: n == 16 NT and panel storage, tiles loaded without bounds tests
 // (load_tile_full).
+constexpr int kSolveGroup = 4;
+
 template <int NT, bool PANEL, bool BF16, bool FULL>
 __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<double> B, Mat<double> X, int n,
                                                          int nrhs, int itmax, double tol, int32_t* info,
@@ -44,7 +46,7 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
     constexpr int NTILES = NT * (NT + 1) / 2;
     constexpr int NV = NT * 16 * kMaxRhs;
     __shared__ __align__(16) float sP[NT][kTileElems];  // block column kb (operand layout); sP[kb]: the diagonal tile
-    __shared__ float sW[kTileElems];      // one staged factor tile in the forward solve
+    __shared__ float sW[kSolveGroup][kTileElems];  // factor tiles staged for the forward solve
     __shared__ double sX[NV];             // x, [row][rhs]
     __shared__ float sV[NV];              // FP32 solve vector, [row][rhs]
 
@@ -136,6 +138,8 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
         for (int r = 0; r < 4; ++r)
             if (cn < kMaxRhs) sV[(16 * blk + M::row(lane, r)) * kMaxRhs + cn] = v[r];
     };
+    // Tiles are processed kSolveGroup at a time, their MFMA chains
+    // interleaved (one chain alone waits on each MFMA's result).
     auto solve = [&]() {
         static_for<0, NT>([&](auto KB) __attribute__((always_inline)) {
             constexpr int kb = decltype(KB)::value;
@@ -143,13 +147,28 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
 #pragma unroll
             for (int q = 0; q < 4; ++q) y = M::mma(op_read(sP[kb], q, lane), vb(kb, 4 * q + (lane >> 4)), y);
             vstore(kb, y);
-            static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
-                constexpr int i = decltype(I)::value;
-                acc_to_lds<float>(sW, acc[tri(i, kb)], lane);
-                acc_t v = vload(i);
+            static_for<0, (NT - 1 - kb + kSolveGroup - 1) / kSolveGroup>([&](auto GI) __attribute__((always_inline)) {
+                constexpr int i0 = kb + 1 + kSolveGroup * decltype(GI)::value;
+                acc_t v[kSolveGroup];
+                static_for<0, kSolveGroup>([&](auto T) __attribute__((always_inline)) {
+                    constexpr int t = decltype(T)::value;
+                    if constexpr (i0 + t < NT) {
+                        acc_to_lds<float>(sW[t], acc[tri(i0 + t, kb)], lane);
+                        v[t] = vload(i0 + t);
+                    }
+                });
 #pragma unroll
-                for (int q = 0; q < 4; ++q) v = M::mma(-op_read(sW, q, lane), vb(kb, 4 * q + (lane >> 4)), v);
-                vstore(i, v);
+                for (int q = 0; q < 4; ++q) {
+                    float const yq = vb(kb, 4 * q + (lane >> 4));
+                    static_for<0, kSolveGroup>([&](auto T) __attribute__((always_inline)) {
+                        constexpr int t = decltype(T)::value;
+                        if constexpr (i0 + t < NT) v[t] = M::mma(-op_read(sW[t], q, lane), yq, v[t]);
+                    });
+                }
+                static_for<0, kSolveGroup>([&](auto T) __attribute__((always_inline)) {
+                    constexpr int t = decltype(T)::value;
+                    if constexpr (i0 + t < NT) vstore(i0 + t, v[t]);
+                });
             });
         });
         static_for<0, NT>([&](auto KK) __attribute__((always_inline)) {
@@ -159,12 +178,25 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
             for (int q = 0; q < 4; ++q)  // A operand W^T: W[row(l, q)][l % 16]
                 x = M::mma(sP[kb][cn * kTileStride + M::row(lane, q)], vb(kb, M::row(lane, q)), x);
             vstore(kb, x);
-            static_for<0, kb>([&](auto J) __attribute__((always_inline)) {
-                constexpr int j = decltype(J)::value;
-                acc_t v = vload(j);
+            static_for<0, (kb + kSolveGroup - 1) / kSolveGroup>([&](auto GI) __attribute__((always_inline)) {
+                constexpr int j0 = kSolveGroup * decltype(GI)::value;
+                acc_t v[kSolveGroup];
+                static_for<0, kSolveGroup>([&](auto T) __attribute__((always_inline)) {
+                    constexpr int t = decltype(T)::value;
+                    if constexpr (j0 + t < kb) v[t] = vload(j0 + t);
+                });
 #pragma unroll
-                for (int q = 0; q < 4; ++q) v = M::mma(-acc[tri(kb, j)][q], vb(kb, M::row(lane, q)), v);
-                vstore(j, v);
+                for (int q = 0; q < 4; ++q) {
+                    float const xq = vb(kb, M::row(lane, q));
+                    static_for<0, kSolveGroup>([&](auto T) __attribute__((always_inline)) {
+                        constexpr int t = decltype(T)::value;
+                        if constexpr (j0 + t < kb) v[t] = M::mma(-acc[tri(kb, j0 + t)][q], xq, v[t]);
+                    });
+                }
+                static_for<0, kSolveGroup>([&](auto T) __attribute__((always_inline)) {
+                    constexpr int t = decltype(T)::value;
+                    if constexpr (j0 + t < kb) vstore(j0 + t, v[t]);
+                });
             });
         });
     };
