@@ -43,6 +43,8 @@ namespace blast_hip {
 // spilled: at kb = 0 the 28 trailing tiles alone fill the accumulator
 // registers.
 constexpr int kReloadCols = 2;
+// Solve tiles processed together, their MFMA chains interleaved.
+constexpr int kSolveGroup = 2;
 
 // FULL: n == 16 NT and panel storage, tiles loaded without bounds tests
 // (load_tile_full).
@@ -150,12 +152,24 @@ __global__ __launch_bounds__(64) void posv_wave_kernel(Mat<double> A, Mat<double
 #pragma unroll
             for (int q = 0; q < 4; ++q) y = M::mma(op_read(sW[kb], q, lane), rk[q], y);
             rstore(kb, y);
-            static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
-                constexpr int i = decltype(I)::value;
-                d4 v = rload(i);
+            // kSolveGroup tiles at a time, their MFMA chains interleaved
+            static_for<0, (NT - 1 - kb + kSolveGroup - 1) / kSolveGroup>([&](auto GI) __attribute__((always_inline)) {
+                constexpr int i0 = kb + 1 + kSolveGroup * decltype(GI)::value;
+                d4 v[kSolveGroup];
+                static_for<0, kSolveGroup>([&](auto T) __attribute__((always_inline)) {
+                    constexpr int t = decltype(T)::value;
+                    if constexpr (i0 + t < NT) v[t] = rload(i0 + t);
+                });
 #pragma unroll
-                for (int q = 0; q < 4; ++q) v = M::mma(-f[i][q], y[q], v);
-                rstore(i, v);
+                for (int q = 0; q < 4; ++q)
+                    static_for<0, kSolveGroup>([&](auto T) __attribute__((always_inline)) {
+                        constexpr int t = decltype(T)::value;
+                        if constexpr (i0 + t < NT) v[t] = M::mma(-f[i0 + t][q], y[q], v[t]);
+                    });
+                static_for<0, kSolveGroup>([&](auto T) __attribute__((always_inline)) {
+                    constexpr int t = decltype(T)::value;
+                    if constexpr (i0 + t < NT) rstore(i0 + t, v[t]);
+                });
             });
         }
         // trailing update: A_ij -= L_ik L_jk^T for kb < j <= i
@@ -184,13 +198,21 @@ __global__ __launch_bounds__(64) void posv_wave_kernel(Mat<double> A, Mat<double
         });
         static_for<0, NT>([&](auto KK) __attribute__((always_inline)) {
             constexpr int kb = NT - 1 - decltype(KK)::value;
-            d4 t = {0.0, 0.0, 0.0, 0.0};
+            // sum_{i>kb} L_ik^T x_i in kSolveGroup partial sums (independent
+            // MFMA chains), added at the end
+            d4 tp[kSolveGroup];
+#pragma unroll
+            for (int g = 0; g < kSolveGroup; ++g) tp[g] = d4{0.0, 0.0, 0.0, 0.0};
             static_for<kb + 1, NT>([&](auto I) __attribute__((always_inline)) {
                 constexpr int i = decltype(I)::value;
                 d4 const xi = rload(i);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) t = M::mma(acc[tri(i, kb)][r], xi[r], t);
+                for (int r = 0; r < 4; ++r)
+                    tp[(i - kb - 1) % kSolveGroup] = M::mma(acc[tri(i, kb)][r], xi[r], tp[(i - kb - 1) % kSolveGroup]);
             });
+            d4 t = tp[0];
+#pragma unroll
+            for (int g = 1; g < kSolveGroup; ++g) t += tp[g];
             d4 const u = rload(kb) - t;
             d4 x = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
