@@ -637,19 +637,19 @@ def main():
     # roofline of the kernel behind `value`; the plain FP64 kernel's beside it
     t_mixed = mixed["ms_per_step"] / 1e3
     roof_plain = {
-        "bound": "hbm", "kernel": "posv_wave_kernel<8, true>",
+        "bound": "hbm", "kernel": "posv_wave_kernel<8, true, true>",
         "achieved": BYTES_PLAIN * batch / t_plain / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
         "frac": BYTES_PLAIN * batch / t_plain / 1e9 / PEAK_HBM_GBS,
-        "traffic": profiled_traffic("posv_wave_kernel<8, true>"), "algorithmic_bytes_per_instance": BYTES_PLAIN,
+        "traffic": profiled_traffic("posv_wave_kernel<8, true, true>"), "algorithmic_bytes_per_instance": BYTES_PLAIN,
         "fp64_tflops": FLOPS_PER_INSTANCE * batch / t_plain / 1e12,
         "fp64_frac_of_peak": FLOPS_PER_INSTANCE * batch / t_plain / 1e12 / PEAK_FP64_TFLOPS,
     }
     if headline == "mixed_fp32_ir":
         roofline = {
-            "bound": "hbm", "kernel": "sposv_ir_kernel<8, true, false>",
+            "bound": "hbm", "kernel": "sposv_ir_kernel<8, true, false, true>",
             "achieved": BYTES_SOLVE_ONLY * batch / t_mixed / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": BYTES_SOLVE_ONLY * batch / t_mixed / 1e9 / PEAK_HBM_GBS,
-            "traffic": profiled_traffic("sposv_ir_kernel<8, true, false>"),
+            "traffic": profiled_traffic("sposv_ir_kernel<8, true, false, true>"),
             "algorithmic_bytes_per_instance": BYTES_SOLVE_ONLY,
             "timed": "one step = the mixed kernel + the FP64 fallback launch (exits at once when nothing fell back)",
             "plain_fp64": roof_plain,
