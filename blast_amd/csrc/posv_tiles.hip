@@ -80,7 +80,9 @@ __global__ __launch_bounds__(64) void posv_wave_kernel(Mat<double> A, Mat<double
         constexpr int i = decltype(I)::value;
         static_for<0, i + 1>([&](auto J) __attribute__((always_inline)) {
             constexpr int j = decltype(J)::value;
-            acc[tri(i, j)] = FULL ? load_tile_full<PANEL, double>(a, i, j, lane) : load_tile<PANEL, double>(a, n, i, j, lane);
+            // (per-element addressing here: with the scalar-addressed L
+            // stores below, the scalar form exceeds the register budget)
+            acc[tri(i, j)] = FULL ? load_tile_full<PANEL, double, false>(a, i, j, lane) : load_tile<PANEL, double>(a, n, i, j, lane);
         });
     });
     if (nrhs > 0)
@@ -128,11 +130,13 @@ __global__ __launch_bounds__(64) void posv_wave_kernel(Mat<double> A, Mat<double
                 z0 = M::mma(op_read(sP[i0], q, lane), w, z0);
                 if constexpr (i1 != i0) z1 = M::mma(op_read(sP[i1], q, lane), w, z1);
             }
-            store_tile<PANEL>(l, lpanel, n, i0, kb, z0, lane);
+            if constexpr (FULL) store_tile_panel_full(l, i0, kb, z0, lane);
+            else store_tile<PANEL>(l, lpanel, n, i0, kb, z0, lane);
             acc_to_lds(sP[i0], z0, lane);
             if constexpr (kb >= kReloadCols) acc[tri(i0, kb)] = z0;
             if constexpr (i1 != i0) {
-                store_tile<PANEL>(l, lpanel, n, i1, kb, z1, lane);
+                if constexpr (FULL) store_tile_panel_full(l, i1, kb, z1, lane);
+                else store_tile<PANEL>(l, lpanel, n, i1, kb, z1, lane);
                 acc_to_lds(sP[i1], z1, lane);
                 if constexpr (kb >= kReloadCols) acc[tri(i1, kb)] = z1;
             }
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(64) void posv_wave_kernel(Mat<double> A, Mat<double
             constexpr int j = decltype(J)::value;
             static_for<j + 1, NT>([&](auto I) __attribute__((always_inline)) {
                 constexpr int i = decltype(I)::value;
-                acc[tri(i, j)] = FULL ? load_tile_full<PANEL, double>(l, i, j, lane) : load_tile<PANEL, double>(l, n, i, j, lane);
+                acc[tri(i, j)] = FULL ? load_tile_full<PANEL, double, false>(l, i, j, lane) : load_tile<PANEL, double>(l, n, i, j, lane);
             });
         });
         static_for<0, NT>([&](auto KK) __attribute__((always_inline)) {

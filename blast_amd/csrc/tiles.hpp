@@ -130,18 +130,43 @@ __device__ __forceinline__ typename Mfma<T>::acc_t load_tile(Ref<double> const& 
     return v;
 }
 
+// Panel storage (PS = 4) address of element (16 ti + row(l, r), 16 tj + l%16)
+// of an accumulator-layout tile, split into a per-lane part (the same for
+// every tile) and a part that is uniform over the wavefront (scalar
+// arithmetic):  p + (4 ti + row/4) * spacing + row % 4 + 4 (16 tj + l % 16)
+//   FP64 (row = l/16 + 4r):   lane (l/16) + 4 (l%16);          uniform (4 ti + r) spacing + 64 tj
+//   FP32 (row = 4 (l/16) + r): lane (l/16) spacing + 4 (l%16);  uniform 4 ti spacing + r + 64 tj
+template <typename T>
+__device__ __forceinline__ int panel_lane_off(int lane, int spacing) {
+    if constexpr (std::is_same_v<T, double>)
+        return (lane >> 4) + 4 * (lane & 15);
+    else
+        return (lane >> 4) * spacing + 4 * (lane & 15);
+}
+template <typename T>
+__device__ __forceinline__ int64_t panel_tile_off(int ti, int tj, int r, int spacing) {
+    if constexpr (std::is_same_v<T, double>)
+        return int64_t(4 * ti + r) * spacing + 64 * tj;
+    else
+        return int64_t(4 * ti) * spacing + r + 64 * tj;
+}
+
 // Tile (ti, tj) of A when n is a multiple of 16 (every tile inside the
 // matrix): no bounds to test, so the loads carry no branches and all of them
 // issue before the first use.  Every address is a lower-triangle element
 // that load_tile also reads: the strict upper part of a diagonal tile is
 // read from its mirror (a lower element) and then zeroed.
-template <bool PANEL, typename T>
+// SCALAR_ADDR: panel tiles addressed as scalar base + lane offset (fewer
+// vector instructions; the per-element form can need fewer registers).
+template <bool PANEL, typename T, bool SCALAR_ADDR = true>
 __device__ __forceinline__ typename Mfma<T>::acc_t load_tile_full(Ref<double> const& a, int ti, int tj, int lane) {
     typename Mfma<T>::acc_t v;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         int const row = 16 * ti + Mfma<T>::row(lane, r), col = 16 * tj + acc_col(lane);
-        if (ti == tj) {
+        if (PANEL && SCALAR_ADDR && ti != tj) {
+            v[r] = T((a.p + panel_tile_off<T>(ti, tj, r, a.m.spacing))[panel_lane_off<T>(lane, a.m.spacing)]);
+        } else if (ti == tj) {
             double const x = at<PANEL>(a, row >= col ? row : col, row >= col ? col : row);
             v[r] = T(row >= col ? x : 0.0);
         } else {
@@ -149,6 +174,14 @@ __device__ __forceinline__ typename Mfma<T>::acc_t load_tile_full(Ref<double> co
         }
     }
     return v;
+}
+
+// Store FP64 off-diagonal tile (ti, tj), ti > tj, of L in panel storage when
+// the tile lies inside the matrix: no tests, scalar tile addressing.
+__device__ __forceinline__ void store_tile_panel_full(Ref<double> const& l, int ti, int tj, d4 v, int lane) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        (l.p + panel_tile_off<double>(ti, tj, r, l.m.spacing))[panel_lane_off<double>(lane, l.m.spacing)] = v[r];
 }
 
 // Store FP64 tile (ti, tj) of L.  Dense storage: i >= j only.  Panel storage:
