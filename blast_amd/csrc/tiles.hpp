@@ -228,11 +228,12 @@ __device__ __forceinline__ float rsqrt_nr(float d) {
 // the sum of lane i of all four rows), without LDS: v_permlane16_swap
 // exchanges odd and even rows, v_permlane32_swap the two halves.  Every lane
 // adds the same operands in the same order, so all four copies are equal.
-// The swap operands and results pass through empty asm statements: given
+// The swap operands and results pass through empty asm statements (not
+// volatile, so that they do not bound scheduling regions): given
 // one value for both operands, the optimiser folds the two results into one
 // register (and then adds x + x).
 __device__ __forceinline__ unsigned distinct_copy(unsigned x) {
-    asm volatile("" : "+v"(x));
+    asm("" : "+v"(x));
     return x;
 }
 template <bool ROW16>
@@ -319,7 +320,9 @@ __device__ int diag_factor(T* sD, T* sW, int lane) {
     T w[4] = {T(0), T(0), T(0), T(0)};
     static_for<0, 16>([&](auto J) __attribute__((always_inline)) {
         constexpr int j = decltype(J)::value;
-        T const f = (i > j) ? sD[j * kTileStride + i] : T(0);  // l_ij
+        // l_ij for i > j, else 0 (the load is unconditional so that it does
+        // not split the code into basic blocks)
+        T const f = sD[j * kTileStride + i] * ((i > j) ? T(1) : T(0));
         T const sj = (i == j) ? -myr : T(1);
         static_for<0, 4>([&](auto U) __attribute__((always_inline)) {
             constexpr int u = decltype(U)::value;
