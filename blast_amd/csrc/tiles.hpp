@@ -309,7 +309,6 @@ __device__ int diag_factor(T* sD, T* sW, int lane) {
         // rows i < k hold values that are never used (the final write masks
         // them; rows_sum and the broadcasts of rows >= k never read them)
         t[uk] = (g == gk) ? v * r : t[uk];
-        __builtin_amdgcn_sched_barrier(0);
     });
 #pragma unroll
     for (int u = 0; u < 4; ++u) sD[(4 * u + g) * kTileStride + i] = (4 * u + g <= i) ? t[u] : T(0);
@@ -334,7 +333,6 @@ __device__ int diag_factor(T* sD, T* sW, int lane) {
                 w[u] = fma_t(f, row_bcast<j>(w[u]), w[u]);
             }
         });
-        __builtin_amdgcn_sched_barrier(0);
     });
 #pragma unroll
     for (int u = 0; u < 4; ++u) sW[(4 * u + g) * kTileStride + i] = (4 * u + g <= i) ? w[u] : T(0);
