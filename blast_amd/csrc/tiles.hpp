@@ -198,12 +198,10 @@ __device__ __forceinline__ void store_tile(Ref<double> const& l, bool panel, int
 }
 
 // Broadcast lane C of every 16-lane row to the whole row (DPP row_newbcast).
+// FP64: one v_mov_b64_dpp (row_newbcast is a legal 64-bit DPP control).
 template <int C>
 __device__ __forceinline__ double row_bcast(double v) {
-    long long const x = __builtin_bit_cast(long long, v);
-    int const lo = __builtin_amdgcn_mov_dpp(int(x), 0x150 + C, 0xF, 0xF, false);
-    int const hi = __builtin_amdgcn_mov_dpp(int(x >> 32), 0x150 + C, 0xF, 0xF, false);
-    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + C, 0xF, 0xF, false);
 }
 template <int C>
 __device__ __forceinline__ float row_bcast(float v) {
@@ -279,13 +277,16 @@ __device__ __forceinline__ T rows_sum(T x) {
 // non-positive pivot (the same value in every lane).  Branch-free: every
 // lane runs the same instructions; all 64 lanes must be active.
 template <typename T>
-__device__ int diag_factor(T* sD, T* sW, int lane) {
+__device__ __attribute__((noinline)) int diag_factor(T* sD_generic, T* sW_generic, int lane) {
+    // LDS-typed pointers: one generic-to-LDS conversion here rather than one
+    // per access (this function is called, so the callers' LDS is not visible)
+    auto* const sD = (__attribute__((address_space(3))) T*)sD_generic;
+    auto* const sW = (__attribute__((address_space(3))) T*)sW_generic;
     int const i = lane & 15, g = lane >> 4;
     T t[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) t[u] = sD[(4 * u + g) * kTileStride + i];
-    int bad = 0;
-    T myr = T(0);
+    T myr = T(0), myd = T(1);
     static_for<0, 16>([&](auto K) __attribute__((always_inline)) {
         constexpr int k = decltype(K)::value, gk = k & 3, uk = k >> 2;
         T p0 = (g == gk) ? t[uk] : T(0), p1 = T(0);
@@ -303,40 +304,39 @@ __device__ int diag_factor(T* sD, T* sW, int lane) {
         });
         T const v = rows_sum(k > 4 ? p0 + p1 : p0);
         T const d = row_bcast<k>(v);  // equals v in lane k
-        bad = (bad == 0 && !(d > T(0))) ? k + 1 : bad;
         T const r = rsqrt_nr(d);
         myr = (i == k) ? r : myr;
+        myd = (i == k) ? d : myd;
         // rows i < k hold values that are never used (the final write masks
         // them; rows_sum and the broadcasts of rows >= k never read them)
         t[uk] = (g == gk) ? v * r : t[uk];
     });
 #pragma unroll
     for (int u = 0; u < 4; ++u) sD[(4 * u + g) * kTileStride + i] = (4 * u + g <= i) ? t[u] : T(0);
-    // W row by row: at step j lane j finalises W[j][c] = (delta_jc - acc) / l_jj
-    // for its columns c <= j; lanes i > j then accumulate acc += l_ij W[j][c].
-    // Lane j finalises with one FMA, w = w*s + e: s = -1/l_jj (s = 1 in the
-    // other lanes), e = 1/l_jj in the column c == j; w is 0 for c > j.
-    T w[4] = {T(0), T(0), T(0), T(0)};
+    // W row by row, kept as z_i[c] = acc_i[c] - delta_ic, where acc_i[c] is
+    // the sum of l_im W[m][c] over the rows m < i applied so far; once all are
+    // in, W[i][c] = -z_i[c] r_i (r_i = 1/l_ii).  At step j row j is final and
+    // rows i > j apply it: z_i += l_ij W[j] = (-l_ij r_j) z_j, one FMA per
+    // column with z_j broadcast from lane j.  z_i[c] = 0 for c > i throughout.
+    T z[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) z[u] = (4 * u + g == i) ? T(-1) : T(0);
     static_for<0, 16>([&](auto J) __attribute__((always_inline)) {
         constexpr int j = decltype(J)::value;
-        // l_ij for i > j, else 0 (the load is unconditional so that it does
-        // not split the code into basic blocks)
-        T const f = sD[j * kTileStride + i] * ((i > j) ? T(1) : T(0));
-        T const sj = (i == j) ? -myr : T(1);
+        T const rj = row_bcast<j>(myr);
+        // -l_ij r_j for i > j, else 0 (the load is unconditional so that it
+        // does not split the code into basic blocks)
+        T const f = sD[j * kTileStride + i] * ((i > j) ? -rj : T(0));
         static_for<0, 4>([&](auto U) __attribute__((always_inline)) {
             constexpr int u = decltype(U)::value;
-            if constexpr (4 * u <= j) {
-                if constexpr (j - 4 * u < 4)
-                    w[u] = fma_t(w[u], sj, (i == j && g == j - 4 * u) ? myr : T(0));
-                else
-                    w[u] = w[u] * sj;
-                w[u] = fma_t(f, row_bcast<j>(w[u]), w[u]);
-            }
+            if constexpr (4 * u <= j) z[u] = fma_t(f, row_bcast<j>(z[u]), z[u]);
         });
     });
 #pragma unroll
-    for (int u = 0; u < 4; ++u) sW[(4 * u + g) * kTileStride + i] = (4 * u + g <= i) ? w[u] : T(0);
-    return bad;
+    for (int u = 0; u < 4; ++u) sW[(4 * u + g) * kTileStride + i] = (4 * u + g <= i) ? -z[u] * myr : T(0);
+    // first non-positive (or NaN) pivot: lanes 0..15 hold d_0..d_15
+    unsigned long long const badl = __ballot(lane < 16 && !(myd > T(0)));
+    return badl ? __builtin_ctzll(badl) + 1 : 0;
 }
 
 // Inverse of the 16x16 triangular tile in sA (T[m][k] at k*17 + m; lower,
