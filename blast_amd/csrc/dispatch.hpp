@@ -19,6 +19,7 @@ template <typename T> int launch_getrf_ref(hipStream_t, Mat<T>, int64_t*, int64_
 // gemm_tiles.hip (MFMA tiles, any size and layout)
 template <typename T> int launch_gemm_tiles(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t);
 template <typename T> int launch_syrk_tiles(hipStream_t, T, Mat<T>, T, Mat<T>, Mat<T>, int64_t);
+template <typename T> int launch_syrk2_tiles(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t);
 template <typename T> int launch_trmm_tiles(hipStream_t, int, T, Mat<T>, Mat<T>, Mat<T>, int64_t);
 template <typename T> int launch_upper_times_lower(hipStream_t, T, Mat<T>, Mat<T>, Mat<T>, int64_t);
 template <typename T> int launch_gemm_overwrite(hipStream_t, T, Mat<T>, Mat<T>, Mat<T>, int64_t);

@@ -99,6 +99,13 @@ int launch_syrk_tiles(hipStream_t s, T alpha, Mat<T> A, T beta, Mat<T> C, Mat<T>
     return launch_mode<T, kSyrkLower>(s, alpha, A, transposed(A), beta, C, D, A.cols, batch);
 }
 
+// D = alpha*A*B^T + beta*C, lower triangle stored: the two-operand form of
+// syrkLower (BLASFEO's syrk_ln; B == A gives syrkLower).
+template <typename T>
+int launch_syrk2_tiles(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D, int64_t batch) {
+    return launch_mode<T, kSyrkLower>(s, alpha, A, transposed(B), beta, C, D, A.cols, batch);
+}
+
 // side_right = 0: C = alpha*A*B (A upper); 1: C = alpha*B*A (A lower).
 template <typename T>
 int launch_trmm_tiles(hipStream_t s, int side_right, T alpha, Mat<T> A, Mat<T> B, Mat<T> C, int64_t batch) {
@@ -121,6 +128,7 @@ int launch_upper_times_lower(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, Mat<T> 
 #define INSTANTIATE(T)                                                                              \
     template int launch_gemm_tiles<T>(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t); \
     template int launch_syrk_tiles<T>(hipStream_t, T, Mat<T>, T, Mat<T>, Mat<T>, int64_t);         \
+    template int launch_syrk2_tiles<T>(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t); \
     template int launch_trmm_tiles<T>(hipStream_t, int, T, Mat<T>, Mat<T>, Mat<T>, int64_t);              \
     template int launch_upper_times_lower<T>(hipStream_t, T, Mat<T>, Mat<T>, Mat<T>, int64_t);           \
     template int launch_gemm_overwrite<T>(hipStream_t, T, Mat<T>, Mat<T>, Mat<T>, int64_t);

@@ -321,6 +321,114 @@ int blast_hip_riccati_autotune(void* stream, int horizon, const blast_riccati_in
 int blast_hip_save_tuning(const char* path);
 int blast_hip_load_tuning(const char* path);
 
+/* ---- BLASFEO-format entry points ------------------------------------------
+ * Batched forms of the BLASFEO routines that BLAST wraps in
+ * include/blasfeo/blasfeo_api/ (Gemm.hpp, Syrk.hpp, Potrf.hpp, SyrkPotrf.hpp,
+ * Trmm.hpp), on matrices in BLASFEO's panel-major format: element (i, j) of
+ * an instance at pA[(i/bs)*bs*cn + i%bs + j*bs].  That is BLAST's panel
+ * storage with spacing bs*cn, so each operand becomes a panel view and the
+ * same kernels run.  Names and argument order follow BLASFEO's C API
+ * (blasfeo_dgemm_nt(m, n, k, alpha, sA, ai, aj, sB, bi, bj, beta, sC, ci,
+ * cj, sD, di, dj), ...), with a stream first and the batch count last.
+ * Differences from BLASFEO: the panel height bs must be BLAST's (4 for
+ * double, 8 for float) and row offsets (ai, bi, ...) multiples of bs;
+ * potrf writes 0 in the strict upper part of each bs x bs diagonal block of
+ * D (BLAST's panel contract) where BLASFEO leaves it untouched.  Outputs
+ * must not overlap inputs, except C == D (same matrix and offsets).
+ * Return 0, -k for invalid argument k (1-based, stream = 1), or
+ * BLAST_HIP_ERR_RUNTIME. */
+typedef struct blast_blasfeo_mat_t {
+    void* pA;             /* device pointer to instance 0 (blasfeo_dmat / blasfeo_smat .pA) */
+    int64_t m;            /* rows (.m) */
+    int64_t n;            /* columns (.n) */
+    int64_t cn;           /* padded columns (.cn >= n): panels are bs*cn elements apart */
+    int64_t bs;           /* panel height of the BLASFEO build (D_PS / S_PS) */
+    int64_t batch_stride; /* elements between consecutive instances */
+} blast_blasfeo_mat_t;
+
+/* D := beta*C + alpha*op(A)*op(B), D and C m x n; nn: A m x k, B k x n;
+ * nt: B stored n x k; tn: A stored k x m; tt: both.  BLAST wraps these as
+ * blasfeo::gemm_nn / gemm_nt / gemm_tn (include/blasfeo/blasfeo_api/Gemm.hpp). */
+int blast_hip_blasfeo_dgemm_nn(void* stream, int m, int n, int k, double alpha, const blast_blasfeo_mat_t* sA,
+                               int ai, int aj, const blast_blasfeo_mat_t* sB, int bi, int bj, double beta,
+                               const blast_blasfeo_mat_t* sC, int ci, int cj, const blast_blasfeo_mat_t* sD,
+                               int di, int dj, int64_t batch);
+int blast_hip_blasfeo_dgemm_nt(void* stream, int m, int n, int k, double alpha, const blast_blasfeo_mat_t* sA,
+                               int ai, int aj, const blast_blasfeo_mat_t* sB, int bi, int bj, double beta,
+                               const blast_blasfeo_mat_t* sC, int ci, int cj, const blast_blasfeo_mat_t* sD,
+                               int di, int dj, int64_t batch);
+int blast_hip_blasfeo_dgemm_tn(void* stream, int m, int n, int k, double alpha, const blast_blasfeo_mat_t* sA,
+                               int ai, int aj, const blast_blasfeo_mat_t* sB, int bi, int bj, double beta,
+                               const blast_blasfeo_mat_t* sC, int ci, int cj, const blast_blasfeo_mat_t* sD,
+                               int di, int dj, int64_t batch);
+int blast_hip_blasfeo_dgemm_tt(void* stream, int m, int n, int k, double alpha, const blast_blasfeo_mat_t* sA,
+                               int ai, int aj, const blast_blasfeo_mat_t* sB, int bi, int bj, double beta,
+                               const blast_blasfeo_mat_t* sC, int ci, int cj, const blast_blasfeo_mat_t* sD,
+                               int di, int dj, int64_t batch);
+int blast_hip_blasfeo_sgemm_nn(void* stream, int m, int n, int k, float alpha, const blast_blasfeo_mat_t* sA,
+                               int ai, int aj, const blast_blasfeo_mat_t* sB, int bi, int bj, float beta,
+                               const blast_blasfeo_mat_t* sC, int ci, int cj, const blast_blasfeo_mat_t* sD,
+                               int di, int dj, int64_t batch);
+int blast_hip_blasfeo_sgemm_nt(void* stream, int m, int n, int k, float alpha, const blast_blasfeo_mat_t* sA,
+                               int ai, int aj, const blast_blasfeo_mat_t* sB, int bi, int bj, float beta,
+                               const blast_blasfeo_mat_t* sC, int ci, int cj, const blast_blasfeo_mat_t* sD,
+                               int di, int dj, int64_t batch);
+int blast_hip_blasfeo_sgemm_tn(void* stream, int m, int n, int k, float alpha, const blast_blasfeo_mat_t* sA,
+                               int ai, int aj, const blast_blasfeo_mat_t* sB, int bi, int bj, float beta,
+                               const blast_blasfeo_mat_t* sC, int ci, int cj, const blast_blasfeo_mat_t* sD,
+                               int di, int dj, int64_t batch);
+int blast_hip_blasfeo_sgemm_tt(void* stream, int m, int n, int k, float alpha, const blast_blasfeo_mat_t* sA,
+                               int ai, int aj, const blast_blasfeo_mat_t* sB, int bi, int bj, float beta,
+                               const blast_blasfeo_mat_t* sC, int ci, int cj, const blast_blasfeo_mat_t* sD,
+                               int di, int dj, int64_t batch);
+
+/* D := beta*C + alpha*A*B^T, lower triangle of the m x m D written; A and B
+ * stored m x k.  blasfeo::syrk_ln (include/blasfeo/blasfeo_api/Syrk.hpp). */
+int blast_hip_blasfeo_dsyrk_ln(void* stream, int m, int k, double alpha, const blast_blasfeo_mat_t* sA, int ai,
+                               int aj, const blast_blasfeo_mat_t* sB, int bi, int bj, double beta,
+                               const blast_blasfeo_mat_t* sC, int ci, int cj, const blast_blasfeo_mat_t* sD,
+                               int di, int dj, int64_t batch);
+int blast_hip_blasfeo_ssyrk_ln(void* stream, int m, int k, float alpha, const blast_blasfeo_mat_t* sA, int ai,
+                               int aj, const blast_blasfeo_mat_t* sB, int bi, int bj, float beta,
+                               const blast_blasfeo_mat_t* sC, int ci, int cj, const blast_blasfeo_mat_t* sD,
+                               int di, int dj, int64_t batch);
+
+/* D := chol(C), lower, m x m (C read from its lower triangle; D may be C).
+ * info as blast_hip_dpotrf_batched.  blasfeo::potrf
+ * (include/blasfeo/blasfeo_api/Potrf.hpp, blasfeo_dpotrf_l). */
+int blast_hip_blasfeo_dpotrf_l(void* stream, int m, const blast_blasfeo_mat_t* sC, int ci, int cj,
+                               const blast_blasfeo_mat_t* sD, int di, int dj, int64_t batch, int32_t* info);
+int blast_hip_blasfeo_spotrf_l(void* stream, int m, const blast_blasfeo_mat_t* sC, int ci, int cj,
+                               const blast_blasfeo_mat_t* sD, int di, int dj, int64_t batch, int32_t* info);
+
+/* D := chol(C + A*B^T), lower, m x m; A and B stored m x k.
+ * blasfeo::syrk_potrf (include/blasfeo/blasfeo_api/SyrkPotrf.hpp,
+ * blasfeo_dsyrk_dpotrf_ln): the lower syrk into D, then potrf in place. */
+int blast_hip_blasfeo_dsyrk_dpotrf_ln(void* stream, int m, int k, const blast_blasfeo_mat_t* sA, int ai, int aj,
+                                      const blast_blasfeo_mat_t* sB, int bi, int bj, const blast_blasfeo_mat_t* sC,
+                                      int ci, int cj, const blast_blasfeo_mat_t* sD, int di, int dj, int64_t batch,
+                                      int32_t* info);
+int blast_hip_blasfeo_ssyrk_spotrf_ln(void* stream, int m, int k, const blast_blasfeo_mat_t* sA, int ai, int aj,
+                                      const blast_blasfeo_mat_t* sB, int bi, int bj, const blast_blasfeo_mat_t* sC,
+                                      int ci, int cj, const blast_blasfeo_mat_t* sD, int di, int dj, int64_t batch,
+                                      int32_t* info);
+
+/* rlnn: D := alpha*B*A, A lower n x n (its strict upper part not read);
+ * rutn: D := alpha*B*A^T, A upper n x n (strict lower part not read); B and
+ * D m x n.  blasfeo::trmm_rlnn / trmm_rutn (include/blasfeo/blasfeo_api/Trmm.hpp). */
+int blast_hip_blasfeo_dtrmm_rlnn(void* stream, int m, int n, double alpha, const blast_blasfeo_mat_t* sA, int ai,
+                                 int aj, const blast_blasfeo_mat_t* sB, int bi, int bj,
+                                 const blast_blasfeo_mat_t* sD, int di, int dj, int64_t batch);
+int blast_hip_blasfeo_dtrmm_rutn(void* stream, int m, int n, double alpha, const blast_blasfeo_mat_t* sA, int ai,
+                                 int aj, const blast_blasfeo_mat_t* sB, int bi, int bj,
+                                 const blast_blasfeo_mat_t* sD, int di, int dj, int64_t batch);
+int blast_hip_blasfeo_strmm_rlnn(void* stream, int m, int n, float alpha, const blast_blasfeo_mat_t* sA, int ai,
+                                 int aj, const blast_blasfeo_mat_t* sB, int bi, int bj,
+                                 const blast_blasfeo_mat_t* sD, int di, int dj, int64_t batch);
+int blast_hip_blasfeo_strmm_rutn(void* stream, int m, int n, float alpha, const blast_blasfeo_mat_t* sA, int ai,
+                                 int aj, const blast_blasfeo_mat_t* sB, int bi, int bj,
+                                 const blast_blasfeo_mat_t* sD, int di, int dj, int64_t batch);
+
 #ifdef __cplusplus
 }
 #endif
