@@ -163,7 +163,7 @@ def gemm_workload(steps, warmup, batch=4096, n=64):
     hbm = 4 * n * n * 8 * batch
     return {"gflops": flops / t / 1e9, "instances_per_s": batch / t, "ms_per_step": t * 1e3,
             "hbm_gbs_algorithmic": hbm / t / 1e9, "hbm_frac": hbm / t / 1e9 / PEAK_HBM_GBS,
-            "kernel": "gemm_tile_kernel<double, 0>"}
+            "kernel": "gemm_small_kernel<double>"}
 
 
 def convert_workload(steps, warmup, batch=4096, n=128):

@@ -82,6 +82,73 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(T alpha, Mat<T> A, Mat<T
     }
 }
 
+// Plain gemm for M, N <= 64 and K <= 64: one 256-thread workgroup per
+// instance, wavefront w owning rows 16w..16w+15 of D.  Every HBM load is
+// issued before the first wait: the wavefront's A strip (already in A-operand
+// layout), its C strip (accumulator layout) and a quarter of B, which is
+// staged in LDS once for all four wavefronts (the tile kernel reads all of B
+// in every wavefront, in four rounds).  Zero padding outside K x N keeps the
+// 16 K-chunks branch-free; the MFMA sequence and the epilogue are the tile
+// kernel's, so the results are identical.
+constexpr int kSmallLdb = 64 + 4;  // LDS row stride of staged B (offsets the four row groups' banks)
+
+template <typename T>
+__global__ __launch_bounds__(256) void gemm_small_kernel(T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D) {
+    using M = Mfma<T>;
+    using acc_t = typename M::acc_t;
+    __shared__ T sB[64 * kSmallLdb];  // B[k][n] at k * kSmallLdb + n
+    int64_t const b = blockIdx.x;
+    int const lane = threadIdx.x & 63;
+    int const w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    Ref<T> a(A, b), bb(B, b), c(C, b), d(D, b);
+    int const Mr = D.rows, Nc = D.cols, K = A.cols;
+    int const m = lane & 15, g = lane >> 4;
+
+    // B rows 16w..16w+15: lane l holds row 16w + 4(j % 4) + l % 4, column
+    // 16(j / 4) + l / 4 (panel storage: 64 consecutive elements per load)
+    T bst[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        int const k = 16 * w + 4 * (j & 3) + (lane & 3), n = 16 * (j >> 2) + (lane >> 2);
+        bst[j] = (k < K && n < Nc) ? bb(k, n) : T(0);
+    }
+    int const row = 16 * w + m;  // A-operand row of this lane
+    T av[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        int const k = 4 * q + g;
+        av[q] = (row < Mr && k < K) ? a(row, k) : T(0);
+    }
+    T cv[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int const i = 16 * w + M::row(lane, r), jj = 16 * t + acc_col(lane);
+            cv[t][r] = (i < Mr && jj < Nc) ? c(i, jj) : T(0);
+        }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        int const k = 16 * w + 4 * (j & 3) + (lane & 3), n = 16 * (j >> 2) + (lane >> 2);
+        sB[k * kSmallLdb + n] = bst[j];
+    }
+    __syncthreads();
+    acc_t acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = acc_t{};
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = M::mma(av[q], sB[(4 * q + g) * kSmallLdb + 16 * t + m], acc[t]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int const i = 16 * w + M::row(lane, r), jj = 16 * t + acc_col(lane);
+            if (i < Mr && jj < Nc) d(i, jj) = fma_t(beta, cv[t][r], acc[t][r] * alpha);
+        }
+}
+
 template <typename T, int MODE>
 static int launch_mode(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D, int K, int64_t batch) {
     if (batch == 0 || D.rows == 0 || D.cols == 0) return 0;
@@ -91,6 +158,13 @@ static int launch_mode(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T
 
 template <typename T>
 int launch_gemm_tiles(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D, int64_t batch) {
+    // 64 x 64 blocks with K in (32, 64]: the single-pass kernel (shorter K
+    // would spend most of its 16 fixed K-chunks on padding)
+    if (D.rows <= 64 && D.cols <= 64 && A.cols > 32 && A.cols <= 64) {
+        if (batch == 0) return 0;
+        hipLaunchKernelGGL((gemm_small_kernel<T>), dim3(unsigned(batch)), dim3(256), 0, s, alpha, A, B, beta, C, D);
+        return check_launch("gemm_small_kernel");
+    }
     return launch_mode<T, kGemm>(s, alpha, A, B, beta, C, D, A.cols, batch);
 }
 

@@ -156,7 +156,8 @@ def _lay(rows, cols, dtype, panel, trans=False):
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-14), (np.float32, 2e-6)])
-@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (5, 7, 3), (16, 16, 16), (64, 64, 64), (33, 70, 9), (100, 20, 129)])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (5, 7, 3), (16, 16, 16), (64, 64, 64), (33, 70, 9), (100, 20, 129),
+                                   (48, 64, 40), (64, 17, 33), (20, 64, 64)])
 @pytest.mark.parametrize("ta,tb,panel", [(0, 1, 1), (0, 0, 0), (1, 1, 1), (1, 0, 0)])
 def test_gemm_tiles(dtype, tol, m, n, k, ta, tb, panel):
     rng = np.random.default_rng(m + 31 * n + 7 * k)
