@@ -110,29 +110,31 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(T alpha, Mat<T> A, Mat<
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         int const k = 16 * w + 4 * (j & 3) + (lane & 3), n = 16 * (j >> 2) + (lane >> 2);
-        bst[j] = (k < K && n < Nc) ? bb(k, n) : T(0);
+        bst[j] = (k < K && n < Nc) ? __builtin_nontemporal_load(&bb(k, n)) : T(0);
     }
-    int const row = 16 * w + m;  // A-operand row of this lane
-    T av[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        int const k = 4 * q + g;
-        av[q] = (row < Mr && k < K) ? a(row, k) : T(0);
-    }
-    T cv[4][4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            int const i = 16 * w + M::row(lane, r), jj = 16 * t + acc_col(lane);
-            cv[t][r] = (i < Mr && jj < Nc) ? c(i, jj) : T(0);
-        }
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         int const k = 16 * w + 4 * (j & 3) + (lane & 3), n = 16 * (j >> 2) + (lane >> 2);
         sB[k * kSmallLdb + n] = bst[j];
     }
     __syncthreads();
+    int const row = 16 * w + m;  // A-operand row of this lane
+    T av[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        int const k = 4 * q + g;
+        av[q] = (row < Mr && k < K) ? __builtin_nontemporal_load(&a(row, k)) : T(0);
+    }
+    // C strip: issued after the staging (its registers are the B staging
+    // registers'), in flight while the MFMAs run
+    T cv[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int const i = 16 * w + M::row(lane, r), jj = 16 * t + acc_col(lane);
+            cv[t][r] = (i < Mr && jj < Nc) ? __builtin_nontemporal_load(&c(i, jj)) : T(0);
+        }
     acc_t acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = acc_t{};
@@ -145,7 +147,7 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(T alpha, Mat<T> A, Mat<
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             int const i = 16 * w + M::row(lane, r), jj = 16 * t + acc_col(lane);
-            if (i < Mr && jj < Nc) d(i, jj) = fma_t(beta, cv[t][r], acc[t][r] * alpha);
+            if (i < Mr && jj < Nc) __builtin_nontemporal_store(fma_t(beta, cv[t][r], acc[t][r] * alpha), &d(i, jj));
         }
 }
 
