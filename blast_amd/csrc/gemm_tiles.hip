@@ -83,13 +83,15 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(T alpha, Mat<T> A, Mat<T
 }
 
 // Plain gemm for M, N <= 64 and K <= 64: one 256-thread workgroup per
-// instance, wavefront w owning rows 16w..16w+15 of D.  Every HBM load is
-// issued before the first wait: the wavefront's A strip (already in A-operand
-// layout), its C strip (accumulator layout) and a quarter of B, which is
-// staged in LDS once for all four wavefronts (the tile kernel reads all of B
-// in every wavefront, in four rounds).  Zero padding outside K x N keeps the
-// 16 K-chunks branch-free; the MFMA sequence and the epilogue are the tile
-// kernel's, so the results are identical.
+// instance, wavefront w owning rows 16w..16w+15 of D.  Each operand is read
+// once: a quarter of B per wavefront, staged in LDS for all four (the tile
+// kernel reads all of B in every wavefront, in four rounds); after the
+// barrier the wavefront's A strip (already in A-operand layout) and C strip
+// (accumulator layout), whose loads overlap the MFMAs (issued this late, they
+// reuse the staging registers: four wavefronts per SIMD).  Every element is
+// touched once, so the accesses are nontemporal.  Zero padding outside K x N
+// keeps the 16 K-chunks branch-free; the MFMA sequence and the epilogue are
+// the tile kernel's, so the results are identical.
 constexpr int kSmallLdb = 64 + 4;  // LDS row stride of staged B (offsets the four row groups' banks)
 
 template <typename T>
