@@ -2,6 +2,7 @@
 // BLAST user would call BLAST: same function names, argument order and
 // exceptions.  The checks are plain loops in this file.  Prints "ALL OK".
 #include <blast/hip/Blast.hpp>
+#include <blast/hip/Blasfeo.hpp>
 
 #include <cmath>
 #include <cstdio>
@@ -278,6 +279,75 @@ int main() {
             CHECK(info[k] == 0);
         }
         CHECK(info[2] == 0);
+    }
+    // BLASFEO-format wrappers (blast/hip/Blasfeo.hpp): gemm_nt on sub-blocks
+    // with exact dyadic data (bit-exact), syrk_potrf, and an argument error.
+    {
+        namespace bf = blast::hip::blasfeo;
+        using BB = bf::BlasfeoBatch<double>;
+        int const batch = 3, m = 10, n = 9, k = 7;
+        std::int64_t const sa = BB::storage(4 + m, k + 3), sb = BB::storage(n, k + 1), sc = BB::storage(m, n),
+                           sd = BB::storage(m + 4, n + 2);
+        std::vector<double> ha(batch * sa, 0.0), hb(batch * sb, 0.0), hc(batch * sc, 0.0), hd(batch * sd, 9.0);
+        BB A0(nullptr, 4 + m, k + 3, k + 3, sa, batch), B0(nullptr, n, k + 1, k + 1, sb, batch),
+            C0(nullptr, m, n, n, sc, batch), D0(nullptr, m + 4, n + 2, n + 2, sd, batch);
+        std::uniform_int_distribution<int> dd(-64, 64);
+        for (auto* v : {&ha, &hb, &hc})
+            for (auto& x : *v) x = dd(g) / 64.0;
+        blast::hip::DeviceArray<double> da(ha.size()), db(hb.size()), dc(hc.size()), dv(hd.size());
+        blast::hip::check_hip(hipMemcpy(da.get(), ha.data(), ha.size() * 8, hipMemcpyHostToDevice));
+        blast::hip::check_hip(hipMemcpy(db.get(), hb.data(), hb.size() * 8, hipMemcpyHostToDevice));
+        blast::hip::check_hip(hipMemcpy(dc.get(), hc.data(), hc.size() * 8, hipMemcpyHostToDevice));
+        blast::hip::check_hip(hipMemcpy(dv.get(), hd.data(), hd.size() * 8, hipMemcpyHostToDevice));
+        BB A = A0, B = B0, C = C0, D = D0;
+        A.pA = da.get(), B.pA = db.get(), C.pA = dc.get(), D.pA = dv.get();
+        // D[4.., 2..] = 0.5*C + 1.5 * A[4.., 3..] * B[0.., 1..]^T
+        bf::gemm_nt(m, n, k, 1.5, A, 4, 3, B, 0, 1, 0.5, C, 0, 0, D, 4, 2);
+        blast::hip::check_hip(hipDeviceSynchronize());
+        std::vector<double> out(hd.size());
+        blast::hip::check_hip(hipMemcpy(out.data(), dv.get(), out.size() * 8, hipMemcpyDeviceToHost));
+        for (int b = 0; b < batch; ++b)
+            for (int i = 0; i < m + 4; ++i)
+                for (int j = 0; j < n + 2; ++j) {
+                    double const got = out[b * sd + D0.index(i, j)];
+                    if (i < 4 || j < 2) {
+                        CHECK(got == 9.0);
+                        continue;
+                    }
+                    double s = 0.0;
+                    for (int l = 0; l < k; ++l) s += ha[b * sa + A0.index(i, 3 + l)] * hb[b * sb + B0.index(j - 2, 1 + l)];
+                    CHECK(got == 0.5 * hc[b * sc + C0.index(i - 4, j - 2)] + 1.5 * s);
+                }
+        // syrk_potrf: D = chol(S + A A^T) with S = m I, A = the first k columns of A's rows 4..
+        int const mm = 8, kk = 5;
+        std::int64_t const ss = BB::storage(mm, mm);
+        std::vector<double> hs(batch * ss, 0.0);
+        BB S0(nullptr, mm, mm, mm, ss, batch);
+        for (int b = 0; b < batch; ++b)
+            for (int i = 0; i < mm; ++i) hs[b * ss + S0.index(i, i)] = double(mm);
+        blast::hip::DeviceArray<double> ds(hs.size()), dl(hs.size());
+        blast::hip::check_hip(hipMemcpy(ds.get(), hs.data(), hs.size() * 8, hipMemcpyHostToDevice));
+        BB S = S0, Lf = S0;
+        S.pA = ds.get(), Lf.pA = dl.get();
+        bf::syrk_potrf(mm, kk, A, 4, 0, A, 4, 0, S, 0, 0, Lf, 0, 0);
+        blast::hip::check_hip(hipDeviceSynchronize());
+        std::vector<double> hl(hs.size());
+        blast::hip::check_hip(hipMemcpy(hl.data(), dl.get(), hl.size() * 8, hipMemcpyDeviceToHost));
+        for (int b = 0; b < batch; ++b)
+            for (int i = 0; i < mm; ++i)
+                for (int j = 0; j <= i; ++j) {
+                    double llt = 0.0, t = i == j ? double(mm) : 0.0;
+                    for (int l = 0; l <= j; ++l) llt += hl[b * ss + S0.index(i, l)] * hl[b * ss + S0.index(j, l)];
+                    for (int l = 0; l < kk; ++l) t += ha[b * sa + A0.index(4 + i, l)] * ha[b * sa + A0.index(4 + j, l)];
+                    CHECK(std::fabs(llt - t) <= 1e-13 * mm);
+                }
+        bool thrown = false;
+        try {
+            bf::potrf(mm, S, 2, 0, Lf, 0, 0);  // row offset 2 splits a panel
+        } catch (std::invalid_argument const&) {
+            thrown = true;
+        }
+        CHECK(thrown);
     }
     if (failures == 0) std::printf("ALL OK\n");
     return failures ? 1 : 0;
