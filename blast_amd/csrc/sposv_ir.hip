@@ -232,6 +232,9 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
 #pragma unroll
                 for (int cc = 0; cc < kMaxRhs; ++cc) r[cc] = cc < nrhs ? bb(rr, cc) : 0.0;
                 double asum = 0.0;
+                // one 16-column batch at a time (unrolling the batches of the
+                // smaller sizes holds several batches' loads and spills)
+#pragma unroll 1
                 for (int c0 = 0; c0 < NT * 16; c0 += 16) {
                     double av[16];
 #pragma unroll
@@ -267,6 +270,9 @@ __global__ __launch_bounds__(64, 2) void sposv_ir_kernel(Mat<double> A, Mat<doub
 #pragma unroll
                 for (int cc = 0; cc < kMaxRhs; ++cc) r[cc] = (row < n && cc < nrhs) ? bb(row, cc) : 0.0;
                 double asum = 0.0;
+                // one 16-column batch at a time (unrolling the batches of the
+                // smaller sizes holds several batches' loads and spills)
+#pragma unroll 1
                 for (int c0 = 0; c0 < NT * 16; c0 += 16) {
                     double av[16];
 #pragma unroll
