@@ -165,7 +165,7 @@ int launch_gemm_tiles(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T>
     // 64 x 64 blocks with K in (32, 64]: the single-pass kernel (shorter K
     // would spend most of its 16 fixed K-chunks on padding)
     if (D.rows <= 64 && D.cols <= 64 && A.cols > 32 && A.cols <= 64) {
-        if (batch == 0) return 0;
+        if (batch == 0 || D.rows == 0 || D.cols == 0) return 0;
         hipLaunchKernelGGL((gemm_small_kernel<T>), dim3(unsigned(batch)), dim3(256), 0, s, alpha, A, B, beta, C, D);
         return check_launch("gemm_small_kernel");
     }

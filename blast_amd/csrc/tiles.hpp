@@ -293,13 +293,15 @@ __device__ __attribute__((noinline)) int diag_factor(T* sD_generic, T* sW_generi
         static_for<0, 4>([&](auto U) __attribute__((always_inline)) {
             constexpr int u = decltype(U)::value;
             if constexpr (4 * u < k) {
-                // column 4u + g: finished (j < k) in every group when 4u + 3 < k
-                T const lij = (4 * u + 3 < k || 4 * u + g < k) ? t[u] : T(0);
-                T const lkj = row_bcast<k>(t[u]);
-                if constexpr (u % 2 == 0)
-                    p0 = fma_t(-lij, lkj, p0);
-                else
-                    p1 = fma_t(-lij, lkj, p1);
+                T& p = u % 2 == 0 ? p0 : p1;
+                T const pr = fma_t(-t[u], row_bcast<k>(t[u]), p);
+                // column 4u + g is finished (j < k) in every group when
+                // 4u + 3 < k; otherwise only where 4u + g < k, the others
+                // holding the diagonal and upper part of rows i and k.  The
+                // product is dropped by a select, not multiplied by 0: a NaN
+                // in the upper part (the full-tile trailing update mirrors
+                // the lower one there) must not reach an earlier pivot.
+                p = (4 * u + 3 < k || 4 * u + g < k) ? pr : p;
             }
         });
         T const v = rows_sum(k > 4 ? p0 + p1 : p0);
