@@ -74,3 +74,24 @@ def test_nan_entry_reported_as_first_bad_pivot(n, panel, i, j):
         got = info.cpu().numpy()
         assert got[2] == i + 1, (solve, got)
         assert np.all(got[[0, 1, 3]] == 0), (solve, got)
+
+
+@pytest.mark.parametrize("n,i,j", [(128, 70, 40), (64, 33, 5)])
+def test_nan_entry_mixed_path_falls_back_and_reports(n, i, j):
+    """dsposv: the FP32 factor of a NaN instance fails, the FP64 fallback
+    reports the first NaN pivot; the other instances refine normally."""
+    import torch
+
+    rng = np.random.default_rng(n * i)
+    batch = 4
+    a = spd(rng, batch, n)
+    a[1, i, j] = a[1, j, i] = np.nan
+    la, lb = Layout.make(n, n), Layout.make(n, 2, panel=False)
+    b = rng.uniform(-1, 1, (batch, n, 2))
+    A, B, X = dev(la, la.pack(a)), dev(lb, lb.pack(b)), capi.Operand.empty(lb, batch)
+    it = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    info = torch.full((batch,), -9, dtype=torch.int32, device="cuda")
+    capi.sposv(0, A, B, X, batch, it, info)
+    its, inf = it.cpu().numpy(), info.cpu().numpy()
+    assert its[1] < 0 and inf[1] == i + 1, (its, inf)
+    assert np.all(its[[0, 2, 3]] >= 0) and np.all(inf[[0, 2, 3]] == 0), (its, inf)
