@@ -154,17 +154,44 @@ __device__ int wg_potrf(DV a, int n, double* sB, double* sW, int* sBad) {
 }
 
 // Solve U X = B in place for upper-triangular U (here U = Ls^T as a
-// transposed view), n x nrhs, by a column sweep from the bottom.
-__device__ void wg_trsm_upper(DV U, DV Bm, int n, int nrhs) {
-    for (int k = n - 1; k >= 0; --k) {
-        double const ukk = U(k, k);
-        for (int j = threadIdx.x; j < nrhs; j += kT) Bm(k, j) = Bm(k, j) / ukk;
-        __syncthreads();
-        for (int e = threadIdx.x; e < k * nrhs; e += kT) {
-            int const i = e % k, j = e / k;
-            Bm(i, j) = __builtin_fma(-U(i, k), Bm(k, j), Bm(i, j));
+// transposed view), n x nrhs, by 16-row blocks from the bottom: the
+// diagonal block is inverted in LDS by one wavefront (tri_inverse), applied
+// to its 16 rows on MFMA (16-column tiles over the wavefronts, in place:
+// each tile's rows are read before any is written), and the rows above are
+// updated with one MFMA product.  Four barriers per block instead of two
+// per row of a column sweep.
+__device__ __attribute__((noinline)) void wg_trsm_upper(DV U, DV Bm, int n, int nrhs, double* sA, double* sW) {
+    using Mf = Mfma<double>;
+    int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int r0 = ((n - 1) / 16) * 16; r0 >= 0; r0 -= 16) {
+        int const kb = min(16, n - r0);
+        // diagonal block (upper) -> LDS as T[m][k] at k*17 + m, identity beyond kb
+        for (int e = threadIdx.x; e < 256; e += kT) {
+            int const i = e & 15, j = e >> 4;
+            sA[j * kTileStride + i] = (i < kb && j < kb) ? (i <= j ? U(r0 + i, r0 + j) : 0.0) : (i == j ? 1.0 : 0.0);
         }
         __syncthreads();
+        if (w == 0) tri_inverse<double>(sA, sW, lane, true, false);
+        __syncthreads();
+        // X_b = U_bb^{-1} B_b
+        for (int T0 = w; 16 * T0 < nrhs; T0 += kT / 64) {
+            int const col = 16 * T0 + (lane & 15);
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                int const k = 4 * q + (lane >> 4);
+                double const bv = (k < kb && col < nrhs) ? Bm(r0 + k, col) : 0.0;
+                acc = Mf::mma(op_read(sW, q, lane), bv, acc);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int const i = Mf::row(lane, r);
+                if (i < kb && col < nrhs) Bm(r0 + i, col) = acc[r];
+            }
+        }
+        __syncthreads();
+        // rows above: B_a -= U_ab X_b
+        if (r0 > 0) wg_gemm<1, kPlain>(-1.0, U.sub(0, r0), Bm.sub(r0, 0), 1.0, Bm, Bm, r0, nrhs, kb);
     }
 }
 
@@ -212,7 +239,7 @@ __device__ void riccati_instance(RiccatiInstance const& ins, int horizon, const 
             Kk(i, j) = -Lam(nu + j, i);
         }
         __syncthreads();
-        wg_trsm_upper(Lam.tr(), Kk, nu, nx);
+        wg_trsm_upper(Lam.tr(), Kk, nu, nx, sB, sW);
     }
     DV const L0{out + ins.l, nx, false};
     for (int e = threadIdx.x; e < nx * nx; e += kT) {
