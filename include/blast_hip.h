@@ -6,7 +6,7 @@
  * reference has no FFI: its boundary is C++20 templates instantiated in the
  * caller's translation unit.  This header is what those templates' batched
  * overloads (include/blast/hip/Blast.hpp) call; any other language binds it
- * directly.
+ * directly (plain C types only: ctypes, cgo, JNI and the like).
  *
  * Conventions
  *  - All matrix data pointers are DEVICE pointers (hipMalloc'd or torch CUDA
