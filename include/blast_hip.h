@@ -24,8 +24,8 @@
  *    invalid; BLAST_HIP_ERR_RUNTIME if the HIP runtime reported an error
  *    (blast_hip_last_error() describes it).  Per-instance numerical status
  *    goes to the optional device array `info` (int32 per instance): 0 = ok,
- *    k > 0 = column k (1-based) had a non-positive (potrf) or exactly zero
- *    (getrf) pivot.  Instances are independent: one bad instance never
+ *    k > 0 = column k (1-based) had a pivot that is not positive (potrf:
+ *    <= 0 or NaN) or exactly zero (getrf).  Instances are independent: one bad instance never
  *    affects another.
  *  - Aliasing allowed as in BLAST: C == D (gemm, syrk), X == B (trsm),
  *    L == A (potrf).
