@@ -48,11 +48,21 @@ struct DV {
 
 enum { kPlain = 0, kSyrkAdd = 1, kUpperA = 2 };
 
+// blkdiag(R, Q) read in place (the C operand of the Lam syrk): no separate
+// fill pass over Lam.
+struct BlkDiagRQ {
+    DV r, q;
+    int nu;
+    __device__ double operator()(int i, int j) const {
+        return i < nu ? (j < nu ? r(i, j) : 0.0) : (j >= nu ? q(i - nu, j - nu) : 0.0);
+    }
+};
+
 // D = alpha*op(A)*op(B) + beta*C on 16x16 MFMA tiles, whole workgroup,
 // TN tiles per wavefront strip.  kSyrkAdd: lower triangle of an M x M
 // result only; kUpperA: A masked to its upper triangle.
-template <int TN, int MODE>
-__device__ void wg_gemm(double alpha, DV A, DV B, double beta, DV C, DV D, int M, int N, int K) {
+template <int TN, int MODE, class CV = DV>
+__device__ void wg_gemm(double alpha, DV A, DV B, double beta, CV C, DV D, int M, int N, int K) {
     using Mf = Mfma<double>;
     int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int const m = lane & 15, g = lane >> 4;
@@ -223,13 +233,9 @@ __device__ void riccati_instance(RiccatiInstance const& ins, int horizon, const 
     for (int k = horizon - 1; k >= 0; --k) {
         // M = L^T [B A]
         wg_gemm<TN, kUpperA>(1.0, Lcur.tr(), BA, 0.0, Mm, Mm, nx, nt, nx);
-        // Lam = blkdiag(R, Q) (lower) + M^T M
-        for (int e = threadIdx.x; e < nt * nt; e += kT) {
-            int const i = e % nt, j = e / nt;
-            if (i >= j) Lam(i, j) = (i < nu && j < nu) ? R(i, j) : (i >= nu && j >= nu ? Q(i - nu, j - nu) : 0.0);
-        }
-        __syncthreads();
-        wg_gemm<TN, kSyrkAdd>(1.0, Mm.tr(), Mm, 1.0, Lam, Lam, nt, nt, nx);
+        // Lam = blkdiag(R, Q) (lower) + M^T M, the block diagonal read in
+        // the syrk epilogue
+        wg_gemm<TN, kSyrkAdd>(1.0, Mm.tr(), Mm, 1.0, BlkDiagRQ{R, Q, nu}, Lam, nt, nt, nx);
         int const bk = wg_potrf<TN>(Lam, nt, sB, sW, sBad);
         if (bk && bad == 0) bad = bk;
         // K_k = -Ls^{-T} Lks^T
