@@ -74,7 +74,9 @@ __device__ void wg_gemm(double alpha, DV A, DV B, double beta, CV C, DV D, int M
             d4 acc[TN];
 #pragma unroll
             for (int t = 0; t < TN; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
-            for (int q0 = 0; q0 < KQ; q0 += 4) {  // four K-chunks of loads in flight
+            // four K-chunks of loads in flight; an upper-triangular A has no
+            // entries left of its strip's diagonal (k < 16 R), so kUpperA starts there
+            for (int q0 = MODE == kUpperA ? 4 * R : 0; q0 < KQ; q0 += 4) {
                 double av[4], bv[4][TN];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
