@@ -155,10 +155,16 @@ __device__ int wg_potrf(DV a, int n, double* sB, double* sW, int* sBad) {
             }
         }
         __syncthreads();
-        // trailing update, lower triangle: A22 -= L21 L21^T
+        // trailing update, lower triangle, A22 -= L21 L21^T, in passes of
+        // 32 columns: after an even block only the next block column is
+        // updated (its factorisation needs it); after the odd block the whole
+        // trailing matrix takes both blocks' columns at once (K = 32), so it
+        // is read and written half as often.
         if (rest > 0) {
-            DV const l21 = a.sub(r0, k0), a22 = a.sub(r0, r0);
-            wg_gemm<TN, kSyrkAdd>(-1.0, l21, l21.tr(), 1.0, a22, a22, rest, rest, kb);
+            bool const odd = ((k0 >> 4) & 1) != 0;  // one call site: two would outgrow the inliner's budget
+            DV const l21 = a.sub(r0, odd ? k0 - 16 : k0), a22 = a.sub(r0, r0);
+            wg_gemm<TN, kSyrkAdd>(-1.0, l21, l21.tr(), 1.0, a22, a22, rest, odd ? rest : min(16, rest),
+                                  odd ? 16 + kb : kb);
         }
     }
     __syncthreads();
