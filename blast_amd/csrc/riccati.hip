@@ -156,15 +156,18 @@ __device__ int wg_potrf(DV a, int n, double* sB, double* sW, int* sBad) {
         }
         __syncthreads();
         // trailing update, lower triangle, A22 -= L21 L21^T, in passes of
-        // 32 columns: after an even block only the next block column is
-        // updated (its factorisation needs it); after the odd block the whole
-        // trailing matrix takes both blocks' columns at once (K = 32), so it
-        // is read and written half as often.
+        // four blocks (64 columns): after block gi = 0, 1, 2 of a group only
+        // the next block column is updated, with the group's blocks so far
+        // (its factorisation needs them); after block 3 the whole trailing
+        // matrix takes the group's four blocks at once (K = 64), so it is read
+        // and written a quarter as often.  Each block column receives each
+        // block's contribution once.  (One call site: two would outgrow the
+        // inliner's budget and spill.)
         if (rest > 0) {
-            bool const odd = ((k0 >> 4) & 1) != 0;  // one call site: two would outgrow the inliner's budget
-            DV const l21 = a.sub(r0, odd ? k0 - 16 : k0), a22 = a.sub(r0, r0);
-            wg_gemm<TN, kSyrkAdd>(-1.0, l21, l21.tr(), 1.0, a22, a22, rest, odd ? rest : min(16, rest),
-                                  odd ? 16 + kb : kb);
+            int const gi = (k0 >> 4) & 3;
+            DV const l21 = a.sub(r0, k0 - 16 * gi), a22 = a.sub(r0, r0);
+            wg_gemm<TN, kSyrkAdd>(-1.0, l21, l21.tr(), 1.0, a22, a22, rest, gi == 3 ? rest : min(16, rest),
+                                  16 * gi + kb);
         }
     }
     __syncthreads();
