@@ -49,7 +49,6 @@ __global__ __launch_bounds__(64 * NW, WPE) void chol_owner_kernel(Mat<double> A,
     __shared__ double sD[kTileElems];          // diagonal tile being factored
     __shared__ double sW[kTileElems];          // W = L_kk^{-1} of the current block
     __shared__ double sR[NT * 16 * kOwnRhs];   // right-hand sides -> y -> x, [row][kOwnRhs]
-    __shared__ d4 sPark[S][64];                // the diagonal owner's other tiles during diag_factor
     __shared__ int sBad;
 
     int64_t const b = blockIdx.x;
@@ -109,18 +108,12 @@ __global__ __launch_bounds__(64 * NW, WPE) void chol_owner_kernel(Mat<double> A,
         {  // 1. diagonal tile, by its owner (one copy of the factor code)
             int const td = kb * (kb + 1) / 2 + kb, sd = td / NW;
             if (w == td % NW) {
-                // park the other tiles in LDS: diag_factor needs the registers
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    if (s == sd)
-                        acc_to_lds(sD, acc[s], lane);
-                    else
-                        sPark[s][lane] = acc[s];
-                }
-                int const bk = diag_factor<double>(sD, sW, lane);
+                d4 dt = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                 for (int s = 0; s < S; ++s)
-                    if (s != sd) acc[s] = sPark[s][lane];
+                    if (s == sd) dt = acc[s];
+                acc_to_lds(sD, dt, lane);
+                int const bk = diag_factor_inl<double>(sD, sW, lane);
                 if (lane == 0 && bk && sBad == 0 && 16 * kb + bk <= n) sBad = 16 * kb + bk;
                 store_tile<PANEL>(l, lpanel, n, kb, kb, lds_to_acc<double>(sD, lane), lane);
                 d4 const wv = lds_to_acc<double>(sW, lane);

@@ -1,0 +1,7 @@
+// Team kernels for n <= 128 (chol_team.hpp), compiled on their own.
+#include "chol_team.hpp"
+
+namespace blast_hip {
+template int launch_team<8, 4, 2>(hipStream_t, Mat<double>, Mat<double>, Mat<double>, Mat<double>, int, int, int64_t,
+                                   int32_t*, const int32_t*);
+}  // namespace blast_hip

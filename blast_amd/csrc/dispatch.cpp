@@ -7,6 +7,18 @@
 
 namespace blast_hip {
 
+// Fused FP64 Cholesky (+ solve): the one-wavefront kernel for full-tile
+// panel shapes, the workgroup kernels otherwise.  Returns 1 when no fused
+// kernel covers the shape.
+int launch_chol_fused(hipStream_t s, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int64_t batch,
+                      int32_t* info, const int32_t* only_if) {
+    int r = launch_posv_tiles(s, A, L, B, X, batch, info, only_if);
+    if (r != 1) return r;
+    r = launch_chol_team(s, A, L, B, X, batch, info, only_if);
+    if (r != 1) return r;
+    return launch_chol_owner(s, A, L, B, X, batch, info, only_if);
+}
+
 template <typename T>
 int dispatch_gemm(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D, int64_t batch) {
     if (algorithm() == BLAST_HIP_ALGO_AUTO) return launch_gemm_tiles<T>(s, alpha, A, B, beta, C, D, batch);
@@ -43,10 +55,8 @@ int dispatch_potrf(hipStream_t s, Mat<T> A, Mat<T> L, int64_t batch, int32_t* in
     if constexpr (std::is_same_v<T, double>) {
         if (algorithm() == BLAST_HIP_ALGO_AUTO) {
             Mat<double> none{};
-            int const r = launch_posv_tiles(s, A, L, none, none, batch, info);
+            int const r = launch_chol_fused(s, A, L, none, none, batch, info);
             if (r != 1) return r;
-            int const r2 = launch_chol_owner(s, A, L, none, none, batch, info);
-            if (r2 != 1) return r2;
         }
     }
     return launch_potrf_ref<T>(s, A, L, batch, info);
@@ -68,10 +78,8 @@ template <typename T>
 int dispatch_posv(hipStream_t s, Mat<T> A, Mat<T> L, Mat<T> B, Mat<T> X, int64_t batch, int32_t* info) {
     if constexpr (std::is_same_v<T, double>) {
         if (algorithm() == BLAST_HIP_ALGO_AUTO && X.cols > 0) {
-            int const r = launch_posv_tiles(s, A, L, B, X, batch, info);
+            int const r = launch_chol_fused(s, A, L, B, X, batch, info);
             if (r != 1) return r;
-            int const r2 = launch_chol_owner(s, A, L, B, X, batch, info);
-            if (r2 != 1) return r2;
         }
     }
     int r = launch_potrf_ref<T>(s, A, L, batch, info);
@@ -87,7 +95,7 @@ int dispatch_posv(hipStream_t s, Mat<T> A, Mat<T> L, Mat<T> B, Mat<T> X, int64_t
 int dispatch_dsposv(hipStream_t s, int bf16, int itmax, double tol, Mat<double> A, Mat<double> B, Mat<double> X,
                     int64_t batch, int32_t* info, int32_t* iter) {
     int r = launch_sposv_ir(s, bf16, A, B, X, itmax, tol, batch, info, iter);
-    if (r == 0) return launch_posv_tiles(s, A, A, B, X, batch, info, iter);
+    if (r == 0) return launch_chol_fused(s, A, A, B, X, batch, info, iter);
     if (r != 1) return r;
     r = launch_fill_i32(s, iter, -3, batch);
     if (r) return r;
@@ -102,9 +110,7 @@ int dispatch_dsposv_ws(hipStream_t s, int bf16, int itmax, double tol, Mat<doubl
     if (wg_covers(A.rows, X.cols) && work) {
         int const r = launch_wg_mixed(s, 0, bf16, itmax, tol, A, nullptr, 0, B, X, X.cols, work, batch, info, iter);
         if (r) return r;
-        int const r2 = launch_posv_tiles(s, A, A, B, X, batch, info, iter);
-        if (r2 != 1) return r2;
-        return launch_chol_owner(s, A, A, B, X, batch, info, iter);  // covers every shape wg_covers accepts
+        return launch_chol_fused(s, A, A, B, X, batch, info, iter);  // covers every shape wg_covers accepts
     }
     int const r = launch_fill_i32(s, iter, -3, batch);
     if (r) return r;

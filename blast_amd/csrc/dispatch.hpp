@@ -44,14 +44,24 @@ template <typename T>
 int getrs_backward(hipStream_t, Mat<T> LU, const int64_t* ipiv, int64_t ipiv_stride, Mat<T> X, Mat<T> Xb, Mat<T> Ab,
                    Mat<T> Bb, int64_t batch);
 
-// posv_tiles.hip: returns 1 when the shape is not covered.  only_if: when
-// non-null, only instances b with only_if[b] < 0 are processed.
-int launch_posv_tiles(hipStream_t, Mat<double>, Mat<double>, Mat<double>, Mat<double>, int64_t, int32_t*,
-                      const int32_t* only_if = nullptr);
+// posv_tiles.hip: one wavefront per instance, n = 16 NT <= 128, A and L in
+// panel storage, nrhs <= 4; returns 1 when the shape is not covered.  only_if
+// as launch_chol_team.
+int launch_posv_tiles(hipStream_t, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int64_t batch,
+                      int32_t* info, const int32_t* only_if = nullptr);
+// dispatch.cpp: the first of posv_tiles, chol_team, chol_owner that covers
+// the shape; 1 when none does.
+int launch_chol_fused(hipStream_t, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int64_t batch,
+                      int32_t* info, const int32_t* only_if = nullptr);
 // chol_owner.hip: fused FP64 Cholesky (+ solve) for n <= 256, nrhs <= 8;
-// returns 1 when the shape is not covered.  only_if as launch_posv_tiles.
+// returns 1 when the shape is not covered.  only_if as launch_chol_team.
 int launch_chol_owner(hipStream_t, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int64_t batch,
                       int32_t* info, const int32_t* only_if = nullptr);
+// chol_team.hip: fused FP64 Cholesky (+ solve) for n <= 128, nrhs <= 4, one
+// workgroup per instance; returns 1 when the shape is not covered.  only_if:
+// when non-null, only instances b with only_if[b] < 0 are processed.
+int launch_chol_team(hipStream_t, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int64_t batch,
+                     int32_t* info, const int32_t* only_if = nullptr);
 // sposv_ir.hip: returns 1 when the shape is not covered
 int launch_sposv_ir(hipStream_t, int bf16, Mat<double> A, Mat<double> B, Mat<double> X, int itmax, double tol,
                     int64_t batch, int32_t* info, int32_t* iter);

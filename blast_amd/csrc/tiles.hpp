@@ -269,24 +269,37 @@ __device__ __forceinline__ T rows_sum(T x) {
 //    a_ik - <l_i, l_k> over its own columns j < k (l_kj broadcast from lane
 //    k), the shares are summed across the groups (rows_sum), then
 //    r = 1/sqrt(d), l_kk = d*r, l_ik = (.)*r, kept by the owner of column k.
+//    The pivot arithmetic (reciprocal square root, scaling) is FP64 for FP32
+//    tiles too; l_ik is rounded to FP32 afterwards.
 // 2. W = L_kk^{-1}, lane (g, i) computing W[i][c] for its columns: row j is
-//    finalised at step j and broadcast from lane j to the lanes below it;
-//    the multipliers l_ij come from the factor written back to sD.
+//    finalised at step j and broadcast from lane j to the lanes below it.
+//    The multipliers -l_ij (strict lower part) are parked in sW first and
+//    scaled by 1/l_jj, broadcast from lane j, as they are used; no per-pivot
+//    lane masks are kept, so the body needs few registers (about 30 VGPRs and
+//    24 SGPRs inlined).
 // On exit sD holds L_kk (zeros above the diagonal) and sW holds W, both as
-// T[m][k] at k*17 + m.  Returns 0 or the 1-based tile column of the first
-// non-positive pivot (the same value in every lane).  Branch-free: every
-// lane runs the same instructions; all 64 lanes must be active.
-template <typename T>
-__device__ __attribute__((noinline)) int diag_factor(T* sD_generic, T* sW_generic, int lane) {
+// T[m][k] at k*17 + m; with sW == sD the tile holds W.  Returns 0 or the
+// 1-based tile column of the first pivot that is not positive (d <= 0 or NaN
+// makes l_kk NaN), the same value in every lane.  Branch-free: every lane
+// runs the same instructions; all 64 lanes must be active.
+// hook(integral_constant<int, s>) runs after step s = 0..31 of the
+// factorisation (s < 16: column s of L_kk; s >= 16: row s - 16 of W), so that
+// a caller can interleave independent work (MFMA updates) with the chain.
+struct NoHook {
+    template <typename S>
+    __device__ __forceinline__ void operator()(S) const {}
+};
+
+template <typename T, typename Hook>
+__device__ __forceinline__ int diag_factor_hooked(T* sD_generic, T* sW_generic, int lane, Hook&& hook) {
     // LDS-typed pointers: one generic-to-LDS conversion here rather than one
-    // per access (this function is called, so the callers' LDS is not visible)
+    // per access when the caller's LDS is not visible
     auto* const sD = (__attribute__((address_space(3))) T*)sD_generic;
     auto* const sW = (__attribute__((address_space(3))) T*)sW_generic;
     int const i = lane & 15, g = lane >> 4;
     T t[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) t[u] = sD[(4 * u + g) * kTileStride + i];
-    T myr = T(0), myd = T(1);
     static_for<0, 16>([&](auto K) __attribute__((always_inline)) {
         constexpr int k = decltype(K)::value, gk = k & 3, uk = k >> 2;
         T p0 = (g == gk) ? t[uk] : T(0), p1 = T(0);
@@ -306,15 +319,26 @@ __device__ __attribute__((noinline)) int diag_factor(T* sD_generic, T* sW_generi
         });
         T const v = rows_sum(k > 4 ? p0 + p1 : p0);
         T const d = row_bcast<k>(v);  // equals v in lane k
-        T const r = rsqrt_nr(d);
-        myr = (i == k) ? r : myr;
-        myd = (i == k) ? d : myd;
-        // rows i < k hold values that are never used (the final write masks
+        double const r = rsqrt_nr(double(d));
+        // rows i < k hold values that are never used (the writes below mask
         // them; rows_sum and the broadcasts of rows >= k never read them)
-        t[uk] = (g == gk) ? v * r : t[uk];
+        t[uk] = (g == gk) ? T(double(v) * r) : t[uk];
+        hook(K);
     });
+    // (sW may be sD: L_kk is written and its diagonal read back before the
+    // multipliers overwrite it)
 #pragma unroll
-    for (int u = 0; u < 4; ++u) sD[(4 * u + g) * kTileStride + i] = (4 * u + g <= i) ? t[u] : T(0);
+    for (int u = 0; u < 4; ++u) {
+        int const c = 4 * u + g;
+        sD[c * kTileStride + i] = (c <= i) ? t[u] : T(0);
+    }
+    T const lii = sD[i * kTileStride + i];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        int const c = 4 * u + g;
+        sW[c * kTileStride + i] = (c < i) ? -t[u] : T(0);
+    }
+    T const myr = T(1.0 / double(lii));
     // W row by row, kept as z_i[c] = acc_i[c] - delta_ic, where acc_i[c] is
     // the sum of l_im W[m][c] over the rows m < i applied so far; once all are
     // in, W[i][c] = -z_i[c] r_i (r_i = 1/l_ii).  At step j row j is final and
@@ -325,20 +349,29 @@ __device__ __attribute__((noinline)) int diag_factor(T* sD_generic, T* sW_generi
     for (int u = 0; u < 4; ++u) z[u] = (4 * u + g == i) ? T(-1) : T(0);
     static_for<0, 16>([&](auto J) __attribute__((always_inline)) {
         constexpr int j = decltype(J)::value;
-        T const rj = row_bcast<j>(myr);
-        // -l_ij r_j for i > j, else 0 (the load is unconditional so that it
-        // does not split the code into basic blocks)
-        T const f = sD[j * kTileStride + i] * ((i > j) ? -rj : T(0));
+        T const f = sW[j * kTileStride + i] * row_bcast<j>(myr);  // -l_ij r_j, 0 for i <= j
         static_for<0, 4>([&](auto U) __attribute__((always_inline)) {
             constexpr int u = decltype(U)::value;
             if constexpr (4 * u <= j) z[u] = fma_t(f, row_bcast<j>(z[u]), z[u]);
         });
+        hook(std::integral_constant<int, 16 + j>{});
     });
 #pragma unroll
     for (int u = 0; u < 4; ++u) sW[(4 * u + g) * kTileStride + i] = (4 * u + g <= i) ? -z[u] * myr : T(0);
-    // first non-positive (or NaN) pivot: lanes 0..15 hold d_0..d_15
-    unsigned long long const badl = __ballot(lane < 16 && !(myd > T(0)));
+    // first pivot that is not positive: lanes 0..15 hold l_00..l_15,15
+    unsigned long long const badl = __ballot(lane < 16 && !(double(lii) > 0.0));
     return badl ? __builtin_ctzll(badl) + 1 : 0;
+}
+
+template <typename T>
+__device__ __forceinline__ int diag_factor_inl(T* sD, T* sW, int lane) {
+    return diag_factor_hooked<T>(sD, sW, lane, NoHook{});
+}
+
+// Called form, for kernels whose register budget cannot hold the inlined body.
+template <typename T>
+__device__ __attribute__((noinline)) int diag_factor(T* sD, T* sW, int lane) {
+    return diag_factor_inl<T>(sD, sW, lane);
 }
 
 // Inverse of the 16x16 triangular tile in sA (T[m][k] at k*17 + m; lower,
