@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kWgThreads) void wg_solve_kernel(WgArgs g, Mat<T> F
         __syncthreads();
         if constexpr (!LU) {
             if (w == 0) {
-                int const bk = diag_factor<T>(sP[kb], sW1[kb], lane);
+                int const bk = diag_factor_inl<T>(sP[kb], sW1[kb], lane);
                 if (lane == 0 && bk && sFlag == 0 && r0 + bk <= n) sFlag = r0 + bk;
             }
             __syncthreads();
