@@ -80,6 +80,8 @@ def lib():
         L.blast_hip_dgesv_batched.argtypes = [vp, P, vp, i64, P, P, i64, i32p]
         L.blast_hip_dsgesv_batched.argtypes = [vp, c_int, c_int, ctypes.c_double, P, vp, i64, P, P, vp, i64, i32p,
                                                i32p]
+        L.blast_hip_dsgesv_noalloc_batched.argtypes = [vp, c_int, c_int, ctypes.c_double, P, vp, i64, P, P, i64,
+                                                       i32p, i32p]
         L.blast_hip_riccati_plan.argtypes = [i64, vp, vp, c_int, vp, vp, vp, vp, vp]
         L.blast_hip_driccati_batched.argtypes = [vp, c_int, vp, i64, vp, vp, vp, vp, vp]
         L.blast_hip_riccati_set_tuning.argtypes = [vp]
@@ -253,6 +255,18 @@ def sgesv(precision, A, ipiv, ipiv_stride, B, X, work, batch, iter_, info=None, 
     return check(name, lib().blast_hip_dsgesv_batched(_stream(stream), int(precision), int(itmax), float(tol),
                                                       _ref(A), _info_ptr(ipiv), ipiv_stride, _ref(B), _ref(X),
                                                       _info_ptr(work), batch, _info_ptr(info), _info_ptr(iter_)))
+
+
+def sgesv_noalloc(precision, A, ipiv, ipiv_stride, B, X, batch, iter_, info=None, itmax=30, tol=0.0, stream=None):
+    """sgesv without a work buffer: the FP32 factor lives in the library's
+    device scratch (NOALLOC_SCRATCH_FLOATS floats per GPU)."""
+    name = "blast_hip_dsgesv_noalloc_batched"
+    return check(name, lib().blast_hip_dsgesv_noalloc_batched(_stream(stream), int(precision), int(itmax), float(tol),
+                                                              _ref(A), _info_ptr(ipiv), ipiv_stride, _ref(B), _ref(X),
+                                                              batch, _info_ptr(info), _info_ptr(iter_)))
+
+
+NOALLOC_SCRATCH_FLOATS = 16 << 20
 
 
 def _oref(op):

@@ -3,6 +3,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 
 #include "common.hpp"
 #include "dispatch.hpp"
@@ -344,6 +345,41 @@ int mixed_params(const char* who, int precision, int itmax, double tol) {
 
 using namespace blast_hip;
 
+namespace {
+
+// Device-resident scratch of the allocation-free entry point (the FP32 factor
+// of blast_hip_dsgesv_noalloc_batched): a file-scope device array, present on
+// every GPU the library runs on until the process ends.  Host code uses only
+// the per-device address from hipGetSymbolAddress, never the array's name.
+__device__ __align__(256) float scratch[BLAST_HIP_NOALLOC_SCRATCH_FLOATS];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of the current device's copy of `scratch`, resolved on the first
+// call on that device; null (with the error recorded) on failure.
+float* device_scratch() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error("noalloc: no current HIP device (or device index >= %d)", kMaxDevices);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t const e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
 extern "C" {
 
 int blast_hip_set_algorithm(int algo) {
@@ -449,6 +485,34 @@ int blast_hip_dgesv_batched(void* s, const blast_matrix_t* A, int64_t* ipiv, int
     (void)hipGetLastError();
     return dispatch_dgesv(hipStream_t(s), Mat<double>::from(*A), ipiv, ipiv_stride, Mat<double>::from(*B),
                           Mat<double>::from(*X), batch, info);
+}
+
+int blast_hip_dsgesv_noalloc_batched(void* s, int precision, int itmax, double tol, const blast_matrix_t* A,
+                                     int64_t* ipiv, int64_t ipiv_stride, const blast_matrix_t* B,
+                                     const blast_matrix_t* X, int64_t batch, int32_t* info, int32_t* iter) {
+    // the checks of blast_hip_dsgesv_batched (argument indices of this entry)
+    if (int r = mixed_params("dsgesv_noalloc", precision, itmax, tol)) return r;
+    if (int r = solve_shapes("dsgesv_noalloc", A, B, X, 5, 8, 9)) return r;
+    if (A->trans) { set_error("dsgesv_noalloc: A must be column-major (trans = 0)"); return -5; }
+    if (A->rows > 0 && !ipiv) { set_error("dsgesv_noalloc: null ipiv"); return -6; }
+    if (ipiv_stride < A->rows) { set_error("dsgesv_noalloc: ipiv_stride < n"); return -7; }
+    if (!valid_batch(batch)) return -10;
+    if (batch > 0 && !iter) { set_error("dsgesv_noalloc: iter is required"); return -12; }
+    float* work = nullptr;
+    if (batch > 0 && wg_covers(int(A->rows), int(X->cols))) {
+        int64_t const need = batch * wg_work_elems(int(A->rows));
+        if (need > int64_t(BLAST_HIP_NOALLOC_SCRATCH_FLOATS)) {
+            set_error("dsgesv_noalloc: batch %lld of n = %lld needs %lld scratch floats, more than the %lld available",
+                      (long long)batch, (long long)A->rows, (long long)need,
+                      (long long)BLAST_HIP_NOALLOC_SCRATCH_FLOATS);
+            return -10;
+        }
+        work = device_scratch();
+        if (!work) return BLAST_HIP_ERR_RUNTIME;
+    }
+    (void)hipGetLastError();
+    return dispatch_dsgesv(hipStream_t(s), precision, itmax, tol, Mat<double>::from(*A), ipiv, ipiv_stride,
+                           Mat<double>::from(*B), Mat<double>::from(*X), work, batch, info, iter);
 }
 
 int blast_hip_dsgesv_batched(void* s, int precision, int itmax, double tol, const blast_matrix_t* A, int64_t* ipiv,

@@ -231,6 +231,10 @@ int blast_hip_scopy_batched(void* stream, const blast_matrix_t* S, const blast_m
  * LAPACK's SWORK. */
 int64_t blast_hip_mixed_work_elems(int64_t n);
 
+/* Size, in floats, of the device scratch behind the allocation-free entry
+ * point blast_hip_dsgesv_noalloc_batched (64 MiB). */
+#define BLAST_HIP_NOALLOC_SCRATCH_FLOATS (16ll << 20)
+
 /* As blast_hip_dsposv_batched, for n <= 256 and nrhs <= 8, with the FP32
  * factor in `work` (device, batch * blast_hip_mixed_work_elems(n) floats;
  * unused, may be NULL, when n <= 128 and nrhs <= 4).  LAPACK dsposv. */
@@ -255,6 +259,20 @@ int blast_hip_dgesv_batched(void* stream, const blast_matrix_t* A, int64_t* ipiv
 int blast_hip_dsgesv_batched(void* stream, int precision, int itmax, double tol, const blast_matrix_t* A,
                              int64_t* ipiv, int64_t ipiv_stride, const blast_matrix_t* B, const blast_matrix_t* X,
                              float* work, int64_t batch, int32_t* info, int32_t* iter);
+
+/* Allocation-free form of blast_hip_dsgesv_batched.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The FP32 factor lives in a device array of the library,
+ * BLAST_HIP_NOALLOC_SCRATCH_FLOATS floats (64 MiB) per GPU, held until the
+ * process ends; a call whose batch * blast_hip_mixed_work_elems(n) exceeds it
+ * returns -10 (batch).  Every call on one GPU uses the same array: calls that
+ * overlap on one GPU (other streams, other host threads) race.  Same
+ * arguments as blast_hip_dsgesv_batched without `work` (so batch is argument
+ * 10, info 11, iter 12); the results equal that entry point's. */
+int blast_hip_dsgesv_noalloc_batched(void* stream, int precision, int itmax, double tol, const blast_matrix_t* A,
+                                     int64_t* ipiv, int64_t ipiv_stride, const blast_matrix_t* B,
+                                     const blast_matrix_t* X, int64_t batch, int32_t* info, int32_t* iter);
 
 /* ---- Uneven batched Riccati recursion (LQR / MPC backward pass), FP64 ----
  * No BLAST function is replaced: this composite strings together the entry
