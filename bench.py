@@ -141,8 +141,9 @@ def time_steps(fn, steps, warmup):
 
 
 def gemm_workload(steps, warmup, batch=4096, n=64):
-    """Batched dgemm NT D = 1.5 A B^T - 0.5 C, m = n = k = 64, StaticPanelMatrix
-    layout, U(-1,1) inputs (the BLAST panel gemm benchmark shape at n = 64)."""
+    """Batched dgemm NN D = 1.5 A B - 0.5 C, m = n = k = 64, every operand in
+    StaticPanelMatrix layout, U(-1,1) inputs (the BLAST panel gemm benchmark
+    shape at n = 64)."""
     import torch
 
     from blast_amd import capi
@@ -153,10 +154,9 @@ def gemm_workload(steps, warmup, batch=4096, n=64):
     bufs = [torch.rand(batch * la.batch_stride, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
             for _ in range(4)]
     A, B, C, D = (capi.Operand(la, t) for t in bufs)
-    Bt = B.t()
 
     def step():
-        capi.gemm(1.5, A, Bt, -0.5, C, D, batch)
+        capi.gemm(1.5, A, B, -0.5, C, D, batch)
 
     t = time_steps(step, steps, warmup)
     flops = 2 * n ** 3 * batch

@@ -80,6 +80,8 @@ def lib():
         L.blast_hip_dgesv_batched.argtypes = [vp, P, vp, i64, P, P, i64, i32p]
         L.blast_hip_dsgesv_batched.argtypes = [vp, c_int, c_int, ctypes.c_double, P, vp, i64, P, P, vp, i64, i32p,
                                                i32p]
+        L.blast_hip_dtrsv_batched.argtypes = [vp, c_int, P, P, P, i64]
+        L.blast_hip_strsv_batched.argtypes = [vp, c_int, P, P, P, i64]
         L.blast_hip_dsgesv_noalloc_batched.argtypes = [vp, c_int, c_int, ctypes.c_double, P, vp, i64, P, P, i64,
                                                        i32p, i32p]
         L.blast_hip_riccati_plan.argtypes = [i64, vp, vp, c_int, vp, vp, vp, vp, vp]
@@ -147,6 +149,11 @@ class Operand:
     def t(self) -> "Operand":
         return Operand(self.layout.t(), self.tensor)
 
+    def sub(self, i, j, m, n) -> "Operand":
+        """Rows i..i+m-1, columns j..j+n-1 of every instance (submatrix view)."""
+        lay, off = self.layout.submatrix(i, j, m, n)
+        return Operand(lay, self.tensor[off:])
+
     def desc(self) -> BlastMatrix:
         return self.layout.descriptor(self.tensor.data_ptr())
 
@@ -185,6 +192,13 @@ def trsm(side_right, upper, unit, alpha, A, B, X, batch, stream=None):
     name = f"blast_hip_{p}trsm_batched"
     return check(name, getattr(lib(), name)(_stream(stream), int(side_right), int(upper), int(unit), alpha, _ref(A),
                                             _ref(B), _ref(X), batch))
+
+
+def trsv(upper, A, b, x, batch, stream=None):
+    """A x = b, one right-hand side per instance (b, x: n x 1 operands)."""
+    p = _prefix(x.layout.dtype)
+    name = f"blast_hip_{p}trsv_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), int(upper), _ref(A), _ref(b), _ref(x), batch))
 
 
 def _info_ptr(info):

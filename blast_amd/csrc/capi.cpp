@@ -131,6 +131,23 @@ int trsm_entry(void* stream, int side_right, int upper, int unit, T alpha, const
                             Mat<T>::from(*B), Mat<T>::from(*X), batch);
 }
 
+// trsvLeftLower / trsvLeftUpper (dense/Trsv.hpp:15-68): one right-hand side
+// per instance (b, x: n x 1), non-unit diagonal, on the trsm kernels.
+template <typename T>
+int trsv_entry(void* stream, int upper, const blast_matrix_t* A, const blast_matrix_t* b, const blast_matrix_t* x,
+               int64_t batch) {
+    if (!valid<T>(A, "A")) return -3;
+    if (!valid<T>(b, "b")) return -4;
+    if (!valid<T>(x, "x")) return -5;
+    if (!valid_batch(batch)) return -6;
+    if (b->cols != 1) { set_error("trsv: b must be a single column"); return -4; }
+    if (A->rows != b->rows || A->cols != b->rows) { set_error("trsv: Invalid argument size"); return -3; }
+    if (x->rows != b->rows || x->cols != 1) { set_error("trsv: Invalid argument size (x)"); return -5; }
+    (void)hipGetLastError();
+    return dispatch_trsm<T>(hipStream_t(stream), 0, upper != 0, 0, T(1), Mat<T>::from(*A), Mat<T>::from(*b),
+                            Mat<T>::from(*x), batch);
+}
+
 template <typename T>
 int potrf_entry(void* stream, const blast_matrix_t* A, const blast_matrix_t* L, int64_t batch, int32_t* info) {
     if (!valid<T>(A, "A")) return -2;
@@ -422,6 +439,14 @@ int blast_hip_dtrsm_batched(void* s, int side_right, int upper, int unit, double
 int blast_hip_strsm_batched(void* s, int side_right, int upper, int unit, float alpha, const blast_matrix_t* A,
                             const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch) {
     return trsm_entry<float>(s, side_right, upper, unit, alpha, A, B, X, batch);
+}
+int blast_hip_dtrsv_batched(void* s, int upper, const blast_matrix_t* A, const blast_matrix_t* b,
+                            const blast_matrix_t* x, int64_t batch) {
+    return trsv_entry<double>(s, upper, A, b, x, batch);
+}
+int blast_hip_strsv_batched(void* s, int upper, const blast_matrix_t* A, const blast_matrix_t* b,
+                            const blast_matrix_t* x, int64_t batch) {
+    return trsv_entry<float>(s, upper, A, b, x, batch);
 }
 int blast_hip_dpotrf_batched(void* s, const blast_matrix_t* A, const blast_matrix_t* L, int64_t batch,
                              int32_t* info) {

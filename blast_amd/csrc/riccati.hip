@@ -216,6 +216,11 @@ __device__ __attribute__((noinline)) void wg_trsm_upper(DV U, DV Bm, int n, int 
     }
 }
 
+// info of an instance: 0; c when chol(Q) meets a pivot that is not positive
+// at column c; kRiccatiStepInfo (k + 1) + c when the factorisation of Lam at
+// step k does (column c <= nu: S = R + B^T P B is not positive definite).
+constexpr int kRiccatiStepInfo = 65536;
+
 template <int TN>
 __device__ void riccati_instance(RiccatiInstance const& ins, int horizon, const double* in, double* out, double* work,
                                  double* sB, double* sW, int* sBad, int32_t* info_out) {
@@ -248,7 +253,7 @@ __device__ void riccati_instance(RiccatiInstance const& ins, int horizon, const 
         // the syrk epilogue
         wg_gemm<TN, kSyrkAdd>(1.0, Mm.tr(), Mm, 1.0, BlkDiagRQ{R, Q, nu}, Lam, nt, nt, nx);
         int const bk = wg_potrf<TN>(Lam, nt, sB, sW, sBad);
-        if (bk && bad == 0) bad = bk;
+        if (bk && bad == 0) bad = kRiccatiStepInfo * (k + 1) + bk;  // step k, column bk of Lam
         // K_k = -Ls^{-T} Lks^T
         DV const Kk{out + ins.k + int64_t(k) * nu * nx, nu, false};
         for (int e = threadIdx.x; e < nu * nx; e += kT) {

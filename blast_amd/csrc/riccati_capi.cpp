@@ -27,6 +27,14 @@ int64_t work_elems(int64_t nx, int64_t nu) {
 
 using namespace blast_hip;
 
+namespace {
+// The persistent kernel's work counter is an int32 that every workgroup (at
+// most 2048) bumps once past count; the step index is encoded in info as
+// 65536 (k + 1).
+constexpr int kMaxCount = 0x7fffffff - 4096;
+constexpr int kMaxHorizon = 32767;
+}  // namespace
+
 extern "C" {
 
 int blast_hip_riccati_plan(int64_t count, const int32_t* nx, const int32_t* nu, int horizon,
@@ -73,14 +81,18 @@ int blast_hip_riccati_plan(int64_t count, const int32_t* nx, const int32_t* nu, 
 
 int blast_hip_driccati_batched(void* stream, int horizon, const blast_riccati_instance_t* inst, int64_t count,
                                const double* in, double* out, double* work, int32_t* queue, int32_t* info) {
-    if (horizon < 1) { set_error("riccati: horizon < 1"); return -2; }
-    if (count < 0 || count > 0x7fffffff) { set_error("riccati: count out of range"); return -4; }
+    if (horizon < 1 || horizon > kMaxHorizon) {
+        set_error("riccati: horizon out of range [1, %d]", kMaxHorizon);
+        return -2;
+    }
+    if (count < 0 || count > kMaxCount) { set_error("riccati: count out of range [0, %d]", kMaxCount); return -4; }
     if (count > 0 && (!inst || !in || !out || !work || !queue)) { set_error("riccati: null pointer"); return -3; }
     (void)hipGetLastError();
     return launch_riccati(hipStream_t(stream), horizon, inst, 0, count, in, out, work, queue, g_tuning, info, 0);
 }
 
 int blast_hip_riccati_set_tuning(const int8_t* tn) {
+    if (!tn) { set_error("riccati tuning: null table"); return -1; }
     for (int i = 0; i < 16; ++i)
         if (tn[i] != 1 && tn[i] != 2 && tn[i] != 4) { set_error("riccati tuning: strip width must be 1, 2 or 4"); return -1; }
     std::memcpy(g_tuning.tn, tn, 16);
@@ -88,6 +100,7 @@ int blast_hip_riccati_set_tuning(const int8_t* tn) {
 }
 
 int blast_hip_riccati_get_tuning(int8_t* tn) {
+    if (!tn) { set_error("riccati tuning: null table"); return -1; }
     std::memcpy(tn, g_tuning.tn, 16);
     return 0;
 }
@@ -100,6 +113,8 @@ int blast_hip_riccati_autotune(void* stream, int horizon, const blast_riccati_in
                                double* out, double* work, int32_t* queue_dev, const int32_t* order_host, int reps,
                                float* ms_out) {
     if (count <= 0) return 0;
+    if (horizon < 1 || horizon > kMaxHorizon) { set_error("riccati_autotune: horizon out of range"); return -2; }
+    if (count > kMaxCount) { set_error("riccati_autotune: count out of range [0, %d]", kMaxCount); return -5; }
     if (!inst_host || !order_host || reps < 1) { set_error("riccati_autotune: bad arguments"); return -4; }
     hipStream_t s = hipStream_t(stream);
     hipEvent_t e0, e1;
@@ -168,7 +183,7 @@ int blast_hip_load_tuning(const char* path) {
         if (std::fscanf(f, "%63s", key) != 1 || std::strcmp(key, "riccati_strip_width") != 0) rc = -1;
         for (int i = 0; i < 16 && rc == 0; ++i) {
             int v = 0;
-            if (std::fscanf(f, "%d", &v) != 1) rc = -1;
+            if (std::fscanf(f, "%d", &v) != 1 || (v != 1 && v != 2 && v != 4)) rc = -1;  // checked before narrowing
             tn[i] = int8_t(v);
         }
         if (rc == 0) rc = blast_hip_riccati_set_tuning(tn);

@@ -86,6 +86,21 @@ class Layout:
         return Layout(self.cols, self.rows, self.dtype, self.panel, not self.trans,
                       self.spacing, self.batch_stride)
 
+    def submatrix(self, i, j, m, n):
+        """Sub-block rows i..i+m-1, columns j..j+n-1 of every instance: its
+        Layout (same storage, spacing and batch stride) and the element offset
+        of its (0, 0) from the instance base.  In panel storage the block's
+        first storage row must lie on a panel boundary, as BLAST's
+        PanelSubmatrix requires (views/submatrix/Panel.hpp:96-102)."""
+        if min(i, j, m, n) < 0 or i + m > self.rows or j + n > self.cols:
+            raise ValueError("Invalid submatrix specification")
+        ps = panel_size(self.dtype)
+        r, c = (j, i) if self.trans else (i, j)
+        if self.panel and m > 0 and n > 0 and r % ps:
+            raise ValueError("submatrix of a panel matrix not aligned on a panel boundary")
+        off = (r // ps) * self.spacing + c * ps if self.panel else r + self.spacing * c
+        return Layout(m, n, self.dtype, self.panel, self.trans, self.spacing, self.batch_stride), off
+
     def descriptor(self, ptr: int) -> BlastMatrix:
         return BlastMatrix(ptr, self.rows, self.cols, self.spacing, self.batch_stride,
                            int(self.panel), int(self.trans))

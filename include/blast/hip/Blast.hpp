@@ -89,6 +89,11 @@ public:
         constexpr std::size_t PS = PanelSize_v<T>;
         return PANEL ? r / PS * spacing_ + r % PS + c * PS : r + spacing_ * c;
     }
+    // Row of logical element (i, j) in the (column-major) storage.
+    std::size_t storageRow(std::size_t i, std::size_t j) const noexcept { return SO == columnMajor ? i : j; }
+    // The whole storage block (what the host overloads stage to the device).
+    T* storageData() noexcept { return store_.p; }
+    T const* storageData() const noexcept { return store_.p; }
     T& operator()(std::size_t i, std::size_t j) noexcept { return store_.p[offset(i, j)]; }
     T const& operator()(std::size_t i, std::size_t j) const noexcept { return store_.p[offset(i, j)]; }
 
@@ -158,10 +163,15 @@ template <HostMatrix MT>
 struct Trans {
     MT& m;
     using ElementType = typename MT::ElementType;
+    static constexpr bool isPanel = std::remove_const_t<MT>::isPanel;
     std::size_t rows() const noexcept { return m.columns(); }
     std::size_t columns() const noexcept { return m.rows(); }
     std::size_t storageSize() const noexcept { return m.storageSize(); }
     auto* data() const noexcept { return m.data(); }
+    auto* storageData() const noexcept { return m.storageData(); }
+    std::size_t offset(std::size_t i, std::size_t j) const noexcept { return m.offset(j, i); }
+    std::size_t storageRow(std::size_t i, std::size_t j) const noexcept { return m.storageRow(j, i); }
+    decltype(auto) operator()(std::size_t i, std::size_t j) const noexcept { return m(j, i); }
     blast_matrix_t descriptor(void* device, std::int64_t stride) const noexcept {
         blast_matrix_t d = m.descriptor(device, stride);
         std::swap(d.rows, d.cols);
@@ -176,6 +186,145 @@ Trans<MT> trans(MT& m) {
 template <HostMatrix MT>
 Trans<MT const> trans(MT const& m) {
     return Trans<MT const>{m};
+}
+
+// ------------------------------------------------------------------ views
+// Every view shares its root's storage: offset(i, j) is the element offset of
+// logical element (i, j) from the root storage block, storageRow(i, j) its row
+// in that storage, and descriptor(device, stride) describes the view inside a
+// device copy of the whole root block whose base is `device`.
+
+// submatrix(M, i, j, m, n): rows i..i+m-1, columns j..j+n-1 of M, a
+// non-owning window (views/Submatrix.hpp:22-297, views/submatrix/Panel.hpp:
+// 43-282).  Out-of-range blocks throw std::invalid_argument("Invalid
+// submatrix specification"); in panel storage the block's first storage row
+// must lie on a panel boundary, else std::logic_error, as BLAST's
+// PanelSubmatrix (views/submatrix/Panel.hpp:96-102).
+template <HostMatrix MT>
+struct Submatrix {
+    using Root = std::remove_const_t<MT>;
+    using ElementType = typename MT::ElementType;
+    static constexpr bool isPanel = Root::isPanel;
+    using T = std::remove_const_t<ElementType>;
+
+    Submatrix(MT& mm, std::size_t i, std::size_t j, std::size_t rows, std::size_t cols)
+        : m(mm), i0(i), j0(j), m_(rows), n_(cols) {
+        if (i + rows > mm.rows() || j + cols > mm.columns())
+            throw std::invalid_argument("Invalid submatrix specification");
+        if (isPanel && rows > 0 && cols > 0 && mm.storageRow(i, j) % PanelSize_v<T> != 0)
+            throw std::logic_error("Submatrices of a panel matrix which are not aligned on a panel boundary are "
+                                   "not supported");
+    }
+    std::size_t rows() const noexcept { return m_; }
+    std::size_t columns() const noexcept { return n_; }
+    std::size_t row() const noexcept { return i0; }
+    std::size_t column() const noexcept { return j0; }
+    std::size_t storageSize() const noexcept { return m.storageSize(); }
+    auto* storageData() const noexcept { return m.storageData(); }
+    auto* data() const noexcept { return m.storageData() + offset(0, 0); }
+    std::size_t offset(std::size_t i, std::size_t j) const noexcept { return m.offset(i0 + i, j0 + j); }
+    std::size_t storageRow(std::size_t i, std::size_t j) const noexcept { return m.storageRow(i0 + i, j0 + j); }
+    decltype(auto) operator()(std::size_t i, std::size_t j) const noexcept { return m(i0 + i, j0 + j); }
+    blast_matrix_t descriptor(void* device, std::int64_t stride) const noexcept {
+        blast_matrix_t d = m.descriptor(device, stride);
+        d.data = static_cast<T*>(device) + offset(0, 0);
+        d.rows = std::int64_t(m_), d.cols = std::int64_t(n_);
+        return d;
+    }
+
+    MT& m;
+    std::size_t i0, j0, m_, n_;
+};
+
+template <HostMatrix MT>
+Submatrix<MT> submatrix(MT& m, std::size_t i, std::size_t j, std::size_t rows, std::size_t cols) {
+    return Submatrix<MT>(m, i, j, rows, cols);
+}
+template <HostMatrix MT>
+Submatrix<MT const> submatrix(MT const& m, std::size_t i, std::size_t j, std::size_t rows, std::size_t cols) {
+    return Submatrix<MT const>(m, i, j, rows, cols);
+}
+
+// One row or column of a matrix as a 1 x n (or n x 1) matrix view
+// (views/Row.hpp:27-275, math/PanelMatrix.hpp:29-40).  Where the vector runs
+// across panels' columns of panel storage at a row inside a panel (a row of
+// a column-major panel matrix whose index is not a multiple of the panel
+// size), its elements are panel-size apart: it is described to the kernels
+// as a dense matrix with that spacing.
+template <HostMatrix MT>
+struct VectorView {
+    using Root = std::remove_const_t<MT>;
+    using ElementType = typename MT::ElementType;
+    static constexpr bool isPanel = Root::isPanel;
+    using T = std::remove_const_t<ElementType>;
+
+    VectorView(MT& mm, std::size_t index, bool is_row) : m(mm), k(index), isRow(is_row) {
+        if (index >= (is_row ? mm.rows() : mm.columns())) throw std::invalid_argument("Invalid vector index");
+    }
+    std::size_t rows() const noexcept { return isRow ? 1 : m.rows(); }
+    std::size_t columns() const noexcept { return isRow ? m.columns() : 1; }
+    std::size_t storageSize() const noexcept { return m.storageSize(); }
+    auto* storageData() const noexcept { return m.storageData(); }
+    std::size_t offset(std::size_t i, std::size_t j) const noexcept {
+        return isRow ? m.offset(k, j) : m.offset(i, k);
+    }
+    std::size_t storageRow(std::size_t i, std::size_t j) const noexcept {
+        return isRow ? m.storageRow(k, j) : m.storageRow(i, k);
+    }
+    decltype(auto) operator()(std::size_t i, std::size_t j) const noexcept { return isRow ? m(k, j) : m(i, k); }
+    blast_matrix_t descriptor(void* device, std::int64_t stride) const noexcept {
+        T* const base = static_cast<T*>(device) + offset(0, 0);
+        if (!isPanel || storageRow(0, 0) % PanelSize_v<T> == 0) {  // a window like submatrix()
+            blast_matrix_t d = m.descriptor(device, stride);
+            d.data = base;
+            d.rows = std::int64_t(rows()), d.cols = std::int64_t(columns());
+            return d;
+        }
+        // elements `step` apart: a dense matrix with one storage row (a row
+        // view; a column view is its transpose)
+        std::size_t const len = isRow ? m.columns() : m.rows();
+        std::size_t const step = len > 1 ? (isRow ? offset(0, 1) : offset(1, 0)) - offset(0, 0) : 1;
+        return blast_matrix_t{base, std::int64_t(rows()), std::int64_t(columns()), std::int64_t(step), stride, 0,
+                              isRow ? 0 : 1};
+    }
+
+    MT& m;
+    std::size_t k;
+    bool isRow;
+};
+
+template <HostMatrix MT>
+VectorView<MT> row(MT& m, std::size_t i) {
+    return VectorView<MT>(m, i, true);
+}
+template <HostMatrix MT>
+VectorView<MT const> row(MT const& m, std::size_t i) {
+    return VectorView<MT const>(m, i, true);
+}
+template <HostMatrix MT>
+VectorView<MT> column(MT& m, std::size_t j) {
+    return VectorView<MT>(m, j, false);
+}
+template <HostMatrix MT>
+VectorView<MT const> column(MT const& m, std::size_t j) {
+    return VectorView<MT const>(m, j, false);
+}
+
+// ptr(M, i, j): matrix pointer to element (i, j), the argument of the
+// pointer-form gemm (dense/DynamicMatrixPointer.hpp:245-262,
+// panel/DynamicPanelMatrixPointer.hpp:274-287).
+template <HostMatrix MT>
+struct MatrixPointer {
+    MT& m;
+    std::size_t i, j;
+};
+template <HostMatrix MT>
+MatrixPointer<MT> ptr(MT& m, std::size_t i = 0, std::size_t j = 0) {
+    return MatrixPointer<MT>{m, i, j};
+}
+template <HostMatrix MT>
+MatrixPointer<MT const> ptr(MT const& m, std::size_t i = 0, std::size_t j = 0) {
+    return MatrixPointer<MT const>{m, i, j};
 }
 
 namespace hip {
@@ -219,6 +368,23 @@ struct Batch {
         std::swap(r.desc.rows, r.desc.cols);
         r.desc.trans = !r.desc.trans;
         return r;
+    }
+    // Sub-block rows i..i+m-1, columns j..j+n-1 of every instance (the
+    // batched submatrix(); same alignment rule for panel storage).
+    Batch sub(std::int64_t i, std::int64_t j, std::int64_t m, std::int64_t n) const {
+        if (i < 0 || j < 0 || m < 0 || n < 0 || i + m > desc.rows || j + n > desc.cols)
+            throw std::invalid_argument("Invalid submatrix specification");
+        using E = std::remove_const_t<T>;
+        std::int64_t const r = desc.trans ? j : i, c = desc.trans ? i : j, ps = std::int64_t(PanelSize_v<E>);
+        if (desc.panel && m > 0 && n > 0 && r % ps != 0)
+            throw std::logic_error("Submatrices of a panel matrix which are not aligned on a panel boundary are "
+                                   "not supported");
+        std::int64_t const off = desc.panel ? r / ps * desc.spacing + c * ps : r + desc.spacing * c;
+        Batch out = *this;
+        out.data = data + off;
+        out.desc.data = const_cast<void*>(static_cast<void const*>(out.data));
+        out.desc.rows = m, out.desc.cols = n;
+        return out;
     }
     operator Batch<T const>() const { return Batch<T const>(data, count, desc); }
     std::int64_t stride() const noexcept { return desc.batch_stride; }
@@ -321,6 +487,17 @@ void trsm(Side side, UpLo uplo, bool unit, T alpha, hip::Batch<T const> A, hip::
         hip::check(blast_hip_dtrsm_batched(s, right, up, unit, alpha, &A.desc, &B.desc, &X.desc, X.count));
     else
         hip::check(blast_hip_strsm_batched(s, right, up, unit, alpha, &A.desc, &B.desc, &X.desc, X.count));
+}
+
+// blast::trsvLeftLower / trsvLeftUpper(A, b, x): A x = b with one right-hand
+// side per instance, b and x as n x 1 matrices (dense/Trsv.hpp:15-68).
+template <UpLo UPLO, typename T>
+void trsv(hip::Batch<T const> A, hip::Batch<T const> b, hip::Batch<T> x, hipStream_t s = nullptr) {
+    int const up = UPLO == UpLo::Upper;
+    if constexpr (std::is_same_v<T, double>)
+        hip::check(blast_hip_dtrsv_batched(s, up, &A.desc, &b.desc, &x.desc, x.count));
+    else
+        hip::check(blast_hip_strsv_batched(s, up, &A.desc, &b.desc, &x.desc, x.count));
 }
 
 // blast::potrf(A, L)  (dense/Potrf.hpp:66-106, panel/Potrf.hpp:59-106).
@@ -468,12 +645,12 @@ class Staged {
 public:
     using T = std::remove_const_t<typename std::remove_reference_t<MT>::ElementType>;
     explicit Staged(MT& m) : m_(m), buf_(m.storageSize()) {
-        hip::check_hip(hipMemcpy(buf_.get(), m.data(), m.storageSize() * sizeof(T), hipMemcpyHostToDevice));
+        hip::check_hip(hipMemcpy(buf_.get(), m.storageData(), m.storageSize() * sizeof(T), hipMemcpyHostToDevice));
     }
     hip::Batch<T> batch() const { return hip::Batch<T>(buf_.get(), 1, m_.descriptor(buf_.get(), 0)); }
     operator hip::Batch<T const>() const { return batch(); }
     void copy_back() {
-        hip::check_hip(hipMemcpy(const_cast<T*>(m_.data()), buf_.get(), m_.storageSize() * sizeof(T),
+        hip::check_hip(hipMemcpy(const_cast<T*>(m_.storageData()), buf_.get(), m_.storageSize() * sizeof(T),
                                  hipMemcpyDeviceToHost));
     }
 
@@ -505,6 +682,87 @@ template <class MT1, class MT2, class MT3, class MT4>
 void gemm(MT1 const& A, MT2 const& B, MT3 const& C, MT4& D) {
     using T = std::remove_const_t<typename MT4::ElementType>;
     gemm(T(1), A, B, T(1), C, D);
+}
+
+// Pointer form: the M x N block of D at ptr D gets alpha * (M x K block at
+// ptr A) * (K x N block at ptr B) + beta * (M x N block at ptr C)
+// (algorithm/Gemm.hpp:41-62).
+template <typename ST1, class MA, class MB, typename ST2, class MC, class MD>
+void gemm(std::size_t M, std::size_t N, std::size_t K, ST1 alpha, MatrixPointer<MA> A, MatrixPointer<MB> B, ST2 beta,
+          MatrixPointer<MC> C, MatrixPointer<MD> D) {
+    auto a = submatrix(A.m, A.i, A.j, M, K);
+    auto b = submatrix(B.m, B.i, B.j, K, N);
+    auto c = submatrix(C.m, C.i, C.j, M, N);
+    auto d = submatrix(D.m, D.i, D.j, M, N);
+    gemm(alpha, a, b, beta, c, d);
+}
+
+// C = alpha A B, A triangular (algorithm/Trmm.hpp:260-277): only the upper
+// form is implemented (std::logic_error otherwise, as BLAST).
+template <typename ST, class MT1, class MT2, class MT3>
+    requires HostMatrix<MT1> && HostMatrix<MT2> && HostMatrix<MT3>
+void trmm(ST alpha, MT1 const& A, UpLo uplo, bool diag, MT2 const& B, MT3& C) {
+    using T = std::remove_const_t<typename MT3::ElementType>;
+    std::size_t const M = B.rows(), N = B.columns();
+    if (A.rows() != M || A.columns() != M) throw std::invalid_argument("Matrix sizes do not match");
+    if (C.rows() != M || C.columns() != N) throw std::invalid_argument("Matrix sizes do not match");
+    if (uplo != UpLo::Upper) throw std::logic_error("Lower-triangular matrices support not implemented in trmm()");
+    detail::Staged<MT1 const> a(A);
+    detail::Staged<MT2 const> b(B);
+    detail::Staged<MT3> c(C);
+    trmm<T>(T(alpha), a.batch(), uplo, diag, b.batch(), c.batch());
+    c.copy_back();
+}
+
+// C = alpha B A, A triangular (algorithm/Trmm.hpp:280-310): only the lower
+// form is implemented (std::logic_error otherwise, as BLAST).
+template <typename ST, class MTB, class MTA, class MTC>
+    requires HostMatrix<MTB> && HostMatrix<MTA> && HostMatrix<MTC>
+void trmm(ST alpha, MTB const& B, MTA const& A, UpLo uplo, bool diag, MTC& C) {
+    using T = std::remove_const_t<typename MTC::ElementType>;
+    std::size_t const M = B.rows(), N = B.columns();
+    if (A.rows() != N || A.columns() != N) throw std::invalid_argument("Matrix sizes do not match");
+    if (C.rows() != M || C.columns() != N) throw std::invalid_argument("Matrix sizes do not match");
+    if (uplo != UpLo::Lower) throw std::logic_error("Upper-triangular matrices support not implemented in trmm()");
+    detail::Staged<MTB const> b(B);
+    detail::Staged<MTA const> a(A);
+    detail::Staged<MTC> c(C);
+    trmm<T>(T(alpha), b.batch(), a.batch(), uplo, diag, c.batch());
+    c.copy_back();
+}
+
+namespace detail {
+
+// x = A^{-1} b for one instance, b and x host vectors (x is resized to n).
+template <UpLo UPLO, class MT, typename T>
+void trsv_host(MT const& A, std::vector<T> const& b, std::vector<T>& x) {
+    std::size_t const N = b.size();
+    if (A.rows() != N || A.columns() != N) throw std::invalid_argument("Invalid argument size");  // dense/Trsv.hpp:25
+    x.resize(N);
+    if (N == 0) return;
+    Staged<MT const> a(A);
+    hip::DeviceArray<T> db(N), dx(N);
+    hip::check_hip(hipMemcpy(db.get(), b.data(), N * sizeof(T), hipMemcpyHostToDevice));
+    blast_matrix_t const vb{db.get(), std::int64_t(N), 1, std::int64_t(N), std::int64_t(N), 0, 0};
+    blast_matrix_t const vx{dx.get(), std::int64_t(N), 1, std::int64_t(N), std::int64_t(N), 0, 0};
+    trsv<UPLO, T>(a.batch(), hip::Batch<T const>(db.get(), 1, vb), hip::Batch<T>(dx.get(), 1, vx));
+    hip::check_hip(hipMemcpy(x.data(), dx.get(), N * sizeof(T), hipMemcpyDeviceToHost));
+}
+
+}  // namespace detail
+
+// Solve A x = b, A lower triangular (dense/Trsv.hpp:15-36).
+template <class MT, typename T>
+    requires HostMatrix<MT>
+void trsvLeftLower(MT const& A, std::vector<T> const& b, std::vector<T>& x) {
+    detail::trsv_host<UpLo::Lower>(A, b, x);
+}
+
+// Solve A x = b, A upper triangular (dense/Trsv.hpp:41-68).
+template <class MT, typename T>
+    requires HostMatrix<MT>
+void trsvLeftUpper(MT const& A, std::vector<T> const& b, std::vector<T>& x) {
+    detail::trsv_host<UpLo::Upper>(A, b, x);
 }
 
 template <typename ST1, class MT1, typename ST2, class MT2, class MT3>

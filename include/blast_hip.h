@@ -103,6 +103,16 @@ int blast_hip_strsm_batched(void* stream, int side_right, int upper, int unit, f
                             const blast_matrix_t* A, const blast_matrix_t* B, const blast_matrix_t* X,
                             int64_t batch);
 
+/* trsvLeftLower / trsvLeftUpper (dense/Trsv.hpp:15-68): A x = b with A
+ * n x n lower (upper = 0) or upper (upper = 1) triangular, non-unit, one
+ * right-hand side per instance: b and x are n x 1 descriptors (any storage,
+ * e.g. a row or column of a panel matrix).  x may alias b.  "Invalid argument
+ * size" (-3) when A is not n x n. */
+int blast_hip_dtrsv_batched(void* stream, int upper, const blast_matrix_t* A, const blast_matrix_t* b,
+                            const blast_matrix_t* x, int64_t batch);
+int blast_hip_strsv_batched(void* stream, int upper, const blast_matrix_t* A, const blast_matrix_t* b,
+                            const blast_matrix_t* x, int64_t batch);
+
 /* L := chol(A), lower, M >= N (tall allowed).  Replaces blast::potrf
  * (include/blast/math/dense/Potrf.hpp:66-106 and panel/Potrf.hpp:59-106).
  * Dense L: only i >= j written.  Panel L: the strict upper part of each 4x4
@@ -312,8 +322,11 @@ int blast_hip_riccati_plan(int64_t count, const int32_t* nx, const int32_t* nu, 
  * entries 1..count hold the queue order (a permutation of 0..count-1, e.g.
  * the plan's order); entry 0 is the work counter, reset by a kernel at the
  * start of every call (no memset, so the call can be captured in a hipGraph
- * and replayed).  info (optional DEVICE int32[count]): 0, or the first
- * column (1-based) of a non-positive pivot met in any step's potrf.  Calls
+ * and replayed).  info (optional DEVICE int32[count]): 0; c when chol(Q)
+ * meets a pivot that is not positive at column c (1-based); 65536 (k + 1) + c
+ * when the first such pivot is in the factorisation of
+ * Lam = [[S, .], [., .]] at step k, column c (c <= nu: S = R + B^T P B is not
+ * positive definite).  1 <= horizon <= 32767, count <= 2^31 - 4097.  Calls
  * that share `queue` or `work` must not overlap. */
 int blast_hip_driccati_batched(void* stream, int horizon, const blast_riccati_instance_t* inst, int64_t count,
                                const double* in, double* out, double* work, int32_t* queue, int32_t* info);

@@ -349,6 +349,131 @@ int main() {
         }
         CHECK(thrown);
     }
+    // Views: gemm on submatrices at offsets (dense, and panel with a row offset
+    // on a panel boundary), the pointer form, rows of a panel matrix (inside a
+    // panel: elements panel-size apart), and the panel alignment rule.
+    {
+        std::mt19937 g(11);
+        blast::DynamicPanelMatrix<double> A(40, 30), Cp(36, 28), Dp(36, 28);
+        blast::DynamicMatrix<double> B(30, 25);
+        fill_dyadic(A, g), fill_dyadic(B, g), fill_dyadic(Cp, g), fill_dyadic(Dp, g);
+        blast::DynamicPanelMatrix<double> D0(36, 28);
+        for (std::size_t j = 0; j < 28; ++j)
+            for (std::size_t i = 0; i < 36; ++i) D0(i, j) = Dp(i, j);
+        // D[8..8+20, 3..3+11] = 1.5 A[4..24, 5..14] B[2..11, 7..18] - 0.5 C[8.., 3..]
+        auto d = blast::submatrix(Dp, 8, 3, 20, 11);
+        blast::gemm(1.5, blast::submatrix(A, 4, 5, 20, 9), blast::submatrix(B, 2, 7, 9, 11), -0.5,
+                    blast::submatrix(Cp, 8, 3, 20, 11), d);
+        bool ok = true;
+        for (std::size_t j = 0; j < 28; ++j)
+            for (std::size_t i = 0; i < 36; ++i) {
+                double ref = D0(i, j);
+                if (i >= 8 && i < 28 && j >= 3 && j < 14) {
+                    double acc = 0.0;
+                    for (std::size_t k = 0; k < 9; ++k) acc += A(4 + i - 8, 5 + k) * B(2 + k, 7 + j - 3);
+                    ref = 1.5 * acc - 0.5 * Cp(i, j);
+                }
+                ok = ok && Dp(i, j) == ref;
+            }
+        CHECK(ok);
+        // the same product through the pointer form, into a fresh D
+        blast::DynamicPanelMatrix<double> D2(36, 28);
+        blast::gemm(20, 11, 9, 1.5, blast::ptr(A, 4, 5), blast::ptr(B, 2, 7), -0.5, blast::ptr(Cp, 8, 3),
+                    blast::ptr(D2, 8, 3));
+        ok = true;
+        for (std::size_t j = 3; j < 14; ++j)
+            for (std::size_t i = 8; i < 28; ++i) ok = ok && D2(i, j) == Dp(i, j);
+        CHECK(ok);
+        // row 5 of A (inside a panel) times B[0..30, 0..25] into row 9 of a dense D
+        blast::DynamicMatrix<double> Dr(12, 25), Cr(12, 25);
+        fill_dyadic(Cr, g);
+        auto dr = blast::row(Dr, 9);
+        blast::gemm(1.0, blast::row(A, 5), B, 1.0, blast::row(Cr, 9), dr);
+        ok = true;
+        for (std::size_t j = 0; j < 25; ++j) {
+            double acc = Cr(9, j);
+            for (std::size_t k = 0; k < 30; ++k) acc += A(5, k) * B(k, j);
+            ok = ok && Dr(9, j) == acc && Dr(8, j) == 0.0;
+        }
+        CHECK(ok);
+        bool threw = false;
+        try {
+            (void)blast::submatrix(A, 2, 0, 4, 4);  // row 2 splits a panel
+        } catch (std::logic_error const&) {
+            threw = true;
+        }
+        CHECK(threw);
+        threw = false;
+        try {
+            (void)blast::submatrix(A, 36, 0, 8, 4);
+        } catch (std::invalid_argument const&) {
+            threw = true;
+        }
+        CHECK(threw);
+    }
+
+    // trmm, both matrix forms (algorithm/Trmm.hpp:260-310), and the
+    // unimplemented triangles.
+    {
+        std::mt19937 g(12);
+        blast::DynamicPanelMatrix<double> U(16, 16), Lo(12, 12), B1(16, 12), C1(16, 12), B2(16, 12), C2(16, 12);
+        fill_dyadic(U, g), fill_dyadic(Lo, g), fill_dyadic(B1, g), fill_dyadic(B2, g);
+        blast::trmm(2.0, U, blast::UpLo::Upper, false, B1, C1);
+        blast::trmm(0.5, B2, Lo, blast::UpLo::Lower, false, C2);
+        bool ok = true;
+        for (std::size_t i = 0; i < 16; ++i)
+            for (std::size_t j = 0; j < 12; ++j) {
+                double a1 = 0.0, a2 = 0.0;
+                for (std::size_t k = i; k < 16; ++k) a1 += U(i, k) * B1(k, j);
+                for (std::size_t k = j; k < 12; ++k) a2 += B2(i, k) * Lo(k, j);
+                ok = ok && C1(i, j) == 2.0 * a1 && C2(i, j) == 0.5 * a2;
+            }
+        CHECK(ok);
+        bool threw = false;
+        try {
+            blast::trmm(1.0, U, blast::UpLo::Lower, false, B1, C1);
+        } catch (std::logic_error const&) {
+            threw = true;
+        }
+        CHECK(threw);
+    }
+
+    // trsvLeftLower / trsvLeftUpper (dense/Trsv.hpp:15-68): banded dyadic
+    // triangles with unit-magnitude diagonals and integer x0, so b = A x0 is
+    // exact; the solve (inverted 16x16 diagonal tiles on MFMA) returns x0 to
+    // rounding.
+    {
+        std::mt19937 g(13);
+        std::uniform_int_distribution<int> d(-3, 3);
+        for (std::size_t n : {1u, 7u, 40u, 129u}) {
+            blast::DynamicMatrix<double> Al(n, n), Au(n, n);
+            std::vector<double> x0(n), bl(n, 0.0), bu(n, 0.0), xl, xu;
+            for (std::size_t i = 0; i < n; ++i) x0[i] = d(g);
+            for (std::size_t i = 0; i < n; ++i)
+                for (std::size_t j = 0; j < n; ++j) {
+                    double const v = i == j ? ((i % 2) ? -1.0 : 1.0) : (std::abs(int(i) - int(j)) < 3 ? d(g) / 8.0 : 0.0);
+                    Al(i, j) = j <= i ? v : 0.0;
+                    Au(i, j) = j >= i ? v : 0.0;
+                }
+            for (std::size_t i = 0; i < n; ++i)
+                for (std::size_t j = 0; j < n; ++j) bl[i] += Al(i, j) * x0[j], bu[i] += Au(i, j) * x0[j];
+            blast::trsvLeftLower(Al, bl, xl);
+            blast::trsvLeftUpper(Au, bu, xu);
+            double el = 0.0, eu = 0.0;
+            for (std::size_t i = 0; i < n; ++i) el = std::max(el, std::abs(xl[i] - x0[i])), eu = std::max(eu, std::abs(xu[i] - x0[i]));
+            CHECK(xl.size() == n && el < 1e-12 * double(n));
+            CHECK(xu.size() == n && eu < 1e-12 * double(n));
+        }
+        bool threw = false;
+        try {
+            std::vector<double> b(3), x;
+            blast::trsvLeftLower(blast::DynamicMatrix<double>(4, 4), b, x);
+        } catch (std::invalid_argument const&) {
+            threw = true;
+        }
+        CHECK(threw);
+    }
+
     if (failures == 0) std::printf("ALL OK\n");
     return failures ? 1 : 0;
 }

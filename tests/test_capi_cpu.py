@@ -73,3 +73,42 @@ def test_bad_storage_is_rejected():
 def test_empty_batches_are_noops():
     lib = capi.lib()
     assert lib.blast_hip_dgemm_batched(None, 1.0, _d(4, 3), _d(3, 5), 1.0, _d(4, 5), _d(4, 5), 0) == 0
+
+
+def test_riccati_arguments_and_tuning_file_are_checked(tmp_path):
+    """Riccati entry limits (horizon, count: the int32 work counter) and the
+    tuning table: null tables and out-of-range strip widths in a tuning file
+    are rejected (257 must not narrow to 1)."""
+    lib = capi.lib()
+    assert lib.blast_hip_riccati_set_tuning(None) == -1
+    assert lib.blast_hip_riccati_get_tuning(None) == -1
+    fake = ctypes.c_void_p(0x1000)
+    assert lib.blast_hip_driccati_batched(None, 40000, fake, 1, fake, fake, fake, fake, None) == -2
+    assert lib.blast_hip_driccati_batched(None, 10, fake, 0x7fffffff - 10, fake, fake, fake, fake, None) == -4
+    saved = (ctypes.c_int8 * 16)()
+    assert lib.blast_hip_riccati_get_tuning(saved) == 0
+    path = tmp_path / "t.tuning"
+    assert lib.blast_hip_save_tuning(str(path).encode()) == 0
+    text = path.read_text().split("\n")
+    widths = text[1].split()
+    assert widths[0] == "riccati_strip_width" and len(widths) == 17
+    bad = tmp_path / "bad.tuning"
+    bad.write_text(text[0] + "\nriccati_strip_width " + " ".join(["257"] + widths[2:]) + "\n")
+    assert lib.blast_hip_load_tuning(str(bad).encode()) == -1
+    after = (ctypes.c_int8 * 16)()
+    assert lib.blast_hip_riccati_get_tuning(after) == 0
+    assert bytes(after) == bytes(saved)
+    assert lib.blast_hip_load_tuning(str(path).encode()) == 0
+
+
+def test_subblock_alignment_rule():
+    from blast_amd.layout import Layout
+
+    la = Layout.make(40, 30)
+    with pytest.raises(ValueError):
+        la.submatrix(2, 0, 4, 4)  # row 2 splits a panel
+    with pytest.raises(ValueError):
+        la.submatrix(36, 0, 8, 4)
+    la.t().submatrix(0, 4, 5, 8)  # transposed view: the storage row is the logical column
+    with pytest.raises(ValueError):
+        la.t().submatrix(0, 6, 5, 8)

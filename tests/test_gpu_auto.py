@@ -183,6 +183,33 @@ def test_gemm_tiles_exact_dyadic_and_inplace():
     np.testing.assert_array_equal(C.logical(batch), 1.5 * (a @ b) - 0.5 * c)
 
 
+def test_gemm_nn_all_panel_64_config():
+    """The dgemm benchmark shape: D = 1.5 A B - 0.5 C, m = n = k = 64, every
+    operand in StaticPanelMatrix storage (NN), U(-1,1) entries; against the
+    oracle (BLAST's per-element FMA order) within the gemm backward-error
+    bound, and bit-exact on dyadic inputs (every sum exact)."""
+    from tests.numerics import dyadic
+
+    rng = np.random.default_rng(64)
+    batch, m = 8, 64
+    la = Layout.make(m, m)  # panel
+    a, b, c = (rng.uniform(-1, 1, (batch, m, m)) for _ in range(3))
+    ah, bh, ch = la.pack(a), la.pack(b), la.pack(c)
+    dh = np.zeros_like(ah)
+    oracle.gemm(1.5, la, ah, la, bh, -0.5, la, ch, la, dh, batch)
+    D = capi.Operand.empty(la, batch)
+    capi.gemm(1.5, dev(la, ah), dev(la, bh), -0.5, dev(la, ch), D, batch)
+    got, ref = D.logical(batch), la.unpack(dh, batch)
+    err = np.abs(got - ref).max() / (m * np.abs(a).max() * np.abs(b).max())
+    assert err < 4 * EPS64, err
+    # dyadic entries: every product and partial sum is exact, so any
+    # summation order gives the same bits
+    a, b, c = dyadic(rng, (batch, m, m)), dyadic(rng, (batch, m, m)), dyadic(rng, (batch, m, m))
+    D = capi.Operand.empty(la, batch)
+    capi.gemm(1.5, dev(la, la.pack(a)), dev(la, la.pack(b)), -0.5, dev(la, la.pack(c)), D, batch)
+    np.testing.assert_array_equal(D.logical(batch), 1.5 * (a @ b) - 0.5 * c)
+
+
 @pytest.mark.parametrize("m,k", [(1, 1), (9, 4), (37, 11), (64, 64)])
 def test_syrk_tiles(m, k):
     rng = np.random.default_rng(m * k)

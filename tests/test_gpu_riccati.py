@@ -80,6 +80,23 @@ def test_riccati_bad_instance_isolated():
     _check(b, A, B, Q, R, idx=[0, 2])
 
 
+def test_riccati_info_names_the_step_of_a_failed_s():
+    """S = R + B^T P B not positive definite (R = -100 I) in instance 1: the
+    Lam factorisation of the first step (k = horizon - 1) fails at column 1,
+    info = 65536 (k + 1) + 1; the other instances are unaffected."""
+    nx, nu = [12, 20, 9], [3, 5, 4]
+    horizon = 3
+    b, A, B, Q, R = _batch(nx, nu, horizon, seed=8)
+    Rbad = list(R)
+    Rbad[1] = -100.0 * np.eye(nu[1])
+    b.pack(A, B, Q, Rbad)
+    b.run()
+    info = b.info.cpu().numpy()
+    assert info[1] == 65536 * horizon + 1, info
+    assert info[0] == 0 and info[2] == 0
+    _check(b, A, B, Q, R, idx=[0, 2])
+
+
 def test_riccati_graph_replay():
     import torch
 
