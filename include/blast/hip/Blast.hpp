@@ -647,7 +647,11 @@ public:
     explicit Staged(MT& m) : m_(m), buf_(m.storageSize()) {
         hip::check_hip(hipMemcpy(buf_.get(), m.storageData(), m.storageSize() * sizeof(T), hipMemcpyHostToDevice));
     }
-    hip::Batch<T> batch() const { return hip::Batch<T>(buf_.get(), 1, m_.descriptor(buf_.get(), 0)); }
+    // (a view's descriptor points inside the staged block: its data pointer is kept)
+    hip::Batch<T> batch() const {
+        blast_matrix_t const d = m_.descriptor(buf_.get(), 0);
+        return hip::Batch<T>(static_cast<T*>(d.data), 1, d);
+    }
     operator hip::Batch<T const>() const { return batch(); }
     void copy_back() {
         hip::check_hip(hipMemcpy(const_cast<T*>(m_.storageData()), buf_.get(), m_.storageSize() * sizeof(T),
