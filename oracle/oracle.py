@@ -49,6 +49,7 @@ def lib():
             f = getattr(_lib, f"oracle_{p}getf2"); f.argtypes = [M, ctypes.POINTER(ctypes.c_int64)]
             f = getattr(_lib, f"oracle_{p}iamax"); f.argtypes = [i64, ctypes.c_void_p, i64]; f.restype = i64
             f = getattr(_lib, f"oracle_{p}posv"); f.argtypes = [M, M, M, M, i64, i32p]
+        _lib.oracle_dposv_avx2_mt.argtypes = [M, M, M, M, i64, i32p, ctypes.c_int]
         _lib.oracle_dlaswp.argtypes = [M, i64, i64, ctypes.POINTER(ctypes.c_int64)]
         _lib.oracle_dger.argtypes = [ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, M, M]
         _lib.oracle_dposv_mt.argtypes = [M, M, M, M, i64, i32p, ctypes.c_int]
@@ -151,6 +152,16 @@ def posv_mt(la, a, ll, l, lb, b, lx, x, batch, threads=0):
     info = _i32(batch)
     used = lib().oracle_dposv_mt(_desc(la, a), _desc(ll, l), _desc(lb, b), _desc(lx, x), batch,
                                  info.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), threads)
+    return info, used
+
+
+def posv_avx2_mt(la, a, ll, l, lb, b, lx, x, batch, threads=0):
+    """BLAST-restated AVX2 Cholesky solve (blast_avx2.cpp), one instance per
+    worker; returns (info, workers)."""
+    info = _i32(batch)
+    used = lib().oracle_dposv_avx2_mt(_desc(la, a), _desc(ll, l), _desc(lb, b), _desc(lx, x), batch,
+                                      info.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), threads)
+    assert used > 0, "unsupported shape for the AVX2 restatement"
     return info, used
 
 
