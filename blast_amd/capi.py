@@ -80,6 +80,8 @@ def lib():
         L.blast_hip_dgesv_batched.argtypes = [vp, P, vp, i64, P, P, i64, i32p]
         L.blast_hip_dsgesv_batched.argtypes = [vp, c_int, c_int, ctypes.c_double, P, vp, i64, P, P, vp, i64, i32p,
                                                i32p]
+        L.blast_hip_uneven_order.argtypes = [i64, vp, vp]
+        L.blast_hip_dposv_uneven_batched.argtypes = [vp, vp, i64, vp, vp]
         L.blast_hip_dtrsv_batched.argtypes = [vp, c_int, P, P, P, i64]
         L.blast_hip_strsv_batched.argtypes = [vp, c_int, P, P, P, i64]
         L.blast_hip_dsgesv_noalloc_batched.argtypes = [vp, c_int, c_int, ctypes.c_double, P, vp, i64, P, P, i64,
@@ -199,6 +201,77 @@ def trsv(upper, A, b, x, batch, stream=None):
     p = _prefix(x.layout.dtype)
     name = f"blast_hip_{p}trsv_batched"
     return check(name, getattr(lib(), name)(_stream(stream), int(upper), _ref(A), _ref(b), _ref(x), batch))
+
+
+class SolveInstance(ctypes.Structure):
+    """blast_solve_instance_t: descriptors of one instance of an uneven batch."""
+    _fields_ = [("a", BlastMatrix), ("l", BlastMatrix), ("b", BlastMatrix), ("x", BlastMatrix)]
+
+
+class UnevenBatch:
+    """Instances of different n packed back to back in one FP64 buffer per
+    operand (A, L, B, X), each in its own Layout; the device descriptor array
+    and the queue (order by blast_hip_uneven_order) are built once."""
+
+    def __init__(self, layouts_a, layouts_b=None, alias_l=False, device="cuda"):
+        import torch
+
+        self.la = list(layouts_a)
+        self.lb = list(layouts_b) if layouts_b is not None else None
+        count = len(self.la)
+        self.count = count
+
+        def offsets(lays):
+            off, o = [], 0
+            for lay in lays:
+                off.append(o)
+                o += lay.batch_stride
+            return off, max(o, 1)
+
+        self.oa, na = offsets(self.la)
+        self.a = torch.zeros(na, dtype=torch.float64, device=device)
+        self.l = self.a if alias_l else torch.zeros(na, dtype=torch.float64, device=device)
+        if self.lb is not None:
+            self.ob, nb = offsets(self.lb)
+            self.b = torch.zeros(nb, dtype=torch.float64, device=device)
+            self.x = torch.zeros(nb, dtype=torch.float64, device=device)
+        inst = (SolveInstance * max(count, 1))()
+        for k in range(count):
+            inst[k].a = self.la[k].descriptor(self.a.data_ptr() + 8 * self.oa[k])
+            inst[k].l = self.la[k].descriptor(self.l.data_ptr() + 8 * self.oa[k])
+            if self.lb is not None:
+                inst[k].b = self.lb[k].descriptor(self.b.data_ptr() + 8 * self.ob[k])
+                inst[k].x = self.lb[k].descriptor(self.x.data_ptr() + 8 * self.ob[k])
+        raw = np.frombuffer(bytes(inst), dtype=np.uint8)
+        self.inst = torch.from_numpy(raw.copy()).to(device)
+        n = np.array([lay.rows for lay in self.la], dtype=np.int32)
+        order = np.zeros(max(count, 1), dtype=np.int32)
+        check("blast_hip_uneven_order", lib().blast_hip_uneven_order(count, n.ctypes.data, order.ctypes.data))
+        self.order = order[:count]
+        self.queue = torch.from_numpy(np.concatenate([[0], self.order]).astype(np.int32)).to(device)
+        self.info = torch.zeros(max(count, 1), dtype=torch.int32, device=device)
+
+    def pack(self, a_list, b_list=None):
+        host = np.zeros(self.a.numel())
+        for k, lay in enumerate(self.la):
+            host[self.oa[k]:self.oa[k] + lay.batch_stride] = lay.pack(a_list[k])
+        self.a.copy_(__import__("torch").from_numpy(host))
+        if b_list is not None:
+            hb = np.zeros(self.b.numel())
+            for k, lay in enumerate(self.lb):
+                hb[self.ob[k]:self.ob[k] + lay.batch_stride] = lay.pack(b_list[k])
+            self.b.copy_(__import__("torch").from_numpy(hb))
+
+    def run(self, stream=None):
+        return check("blast_hip_dposv_uneven_batched",
+                     lib().blast_hip_dposv_uneven_batched(_stream(stream), self.inst.data_ptr(), self.count,
+                                                          self.queue.data_ptr(), self.info.data_ptr()))
+
+    def result(self, which, k):
+        buf, lays, offs = ((self.l, self.la, self.oa) if which == "l" else (self.x, self.lb, self.ob))
+        lay = lays[k]
+        host = buf[offs[k]:offs[k] + lay.batch_stride].cpu().numpy()
+        return lay.unpack(host, 1)[0]
 
 
 def _info_ptr(info):

@@ -1,9 +1,12 @@
 // C ABI entry points: argument validation (BLAST's size checks), algorithm
 // dispatch, error reporting.  No allocation, no synchronisation.
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <mutex>
+#include <numeric>
+#include <vector>
 
 #include "common.hpp"
 #include "dispatch.hpp"
@@ -510,6 +513,28 @@ int blast_hip_dgesv_batched(void* s, const blast_matrix_t* A, int64_t* ipiv, int
     (void)hipGetLastError();
     return dispatch_dgesv(hipStream_t(s), Mat<double>::from(*A), ipiv, ipiv_stride, Mat<double>::from(*B),
                           Mat<double>::from(*X), batch, info);
+}
+
+int blast_hip_uneven_max_n(void) { return uneven_max_n(); }
+
+int blast_hip_uneven_order(int64_t count, const int32_t* n, int32_t* order) {
+    if (count < 0 || count > 0x7fffffff - 4096) { set_error("uneven_order: count out of range"); return -1; }
+    if (count > 0 && (!n || !order)) { set_error("uneven_order: null array"); return -2; }
+    std::vector<int32_t> idx(static_cast<size_t>(count));
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) { return (n[x] + 15) / 16 > (n[y] + 15) / 16; });
+    std::copy(idx.begin(), idx.end(), order);
+    return 0;
+}
+
+int blast_hip_dposv_uneven_batched(void* s, const blast_solve_instance_t* inst, int64_t count, int32_t* queue,
+                                   int32_t* info) {
+    // the work counter bumps once past count per workgroup (at most 2048)
+    if (count < 0 || count > 0x7fffffff - 4096) { set_error("posv_uneven: count out of range"); return -3; }
+    if (count > 0 && !inst) { set_error("posv_uneven: null instance array"); return -2; }
+    if (count > 0 && !queue) { set_error("posv_uneven: null queue"); return -4; }
+    (void)hipGetLastError();
+    return launch_posv_uneven(hipStream_t(s), inst, count, queue, info, 0);
 }
 
 int blast_hip_dsgesv_noalloc_batched(void* s, int precision, int itmax, double tol, const blast_matrix_t* A,

@@ -38,21 +38,33 @@ __device__ __forceinline__ int opaque(int v) {
     return v;
 }
 
-template <int NT, int NW, int WPE, bool PANEL>
-__global__ __launch_bounds__(64 * NW, WPE) void chol_owner_kernel(Mat<double> A, Mat<double> L, Mat<double> B,
-                                                             Mat<double> X, int n, int nrhs, int32_t* info,
-                                                             const int32_t* only_if) {
+// Shared memory of one instance with up to NT block columns.
+template <int NT>
+struct OwnerLds {
+    double sP[NT][kTileElems];     // panel tiles L_ik of the current block, T[m][k]
+    double sD[kTileElems];         // diagonal tile being factored
+    double sW[kTileElems];         // W = L_kk^{-1} of the current block
+    double sR[NT * 16 * kOwnRhs];  // right-hand sides -> y -> x, [row][kOwnRhs]
+    int bad;
+};
+
+// One instance (index b of the batch operands) by the NW wavefronts of the
+// workgroup; `lds` is at least OwnerLds<NT> (a larger one is reused by the
+// uneven kernel, whose instances have different NT).
+template <int NT, int NW, bool PANEL, int NTL>
+__device__ __forceinline__ void chol_owner_body(Mat<double> const& A, Mat<double> const& L, Mat<double> const& B,
+                                                Mat<double> const& X, int n, int nrhs, int64_t b, int32_t* info,
+                                                OwnerLds<NTL>& lds) {
+    static_assert(NT <= NTL);
     using Mf = Mfma<double>;
     constexpr int NTILES = NT * (NT + 1) / 2;
     constexpr int S = (NTILES + NW - 1) / NW;  // register tiles per wavefront
-    __shared__ double sP[NT][kTileElems];      // panel tiles L_ik of the current block, T[m][k]
-    __shared__ double sD[kTileElems];          // diagonal tile being factored
-    __shared__ double sW[kTileElems];          // W = L_kk^{-1} of the current block
-    __shared__ double sR[NT * 16 * kOwnRhs];   // right-hand sides -> y -> x, [row][kOwnRhs]
-    __shared__ int sBad;
+    auto& sP = lds.sP;
+    auto& sD = lds.sD;
+    auto& sW = lds.sW;
+    auto& sR = lds.sR;
+    auto& sBad = lds.bad;
 
-    int64_t const b = blockIdx.x;
-    if (only_if && only_if[b] >= 0) return;
     int const lane = threadIdx.x & 63;
     int const w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     Ref<double> a(A, b), l(L, b);
@@ -200,6 +212,75 @@ __global__ __launch_bounds__(64 * NW, WPE) void chol_owner_kernel(Mat<double> A,
     if (info && threadIdx.x == 0) info[b] = sBad;
 }
 
+template <int NT, int NW, int WPE, bool PANEL>
+__global__ __launch_bounds__(64 * NW, WPE) void chol_owner_kernel(Mat<double> A, Mat<double> L, Mat<double> B,
+                                                             Mat<double> X, int n, int nrhs, int32_t* info,
+                                                             const int32_t* only_if) {
+    __shared__ OwnerLds<NT> lds;
+    int64_t const b = blockIdx.x;
+    if (only_if && only_if[b] >= 0) return;
+    chol_owner_body<NT, NW, PANEL>(A, L, B, X, n, nrhs, b, info, lds);
+}
+
+// Uneven batches: one persistent launch over instances of different n (one
+// workgroup of kUnevenNW wavefronts per instance, n <= 16 kUnevenNT).  Each
+// workgroup takes the next queue entry (atomic counter in queue[0], the
+// order in queue[1..count], largest block count first so that small
+// instances fill the tail) until the queue is empty.
+constexpr int kUnevenNT = 12, kUnevenNW = 4;
+
+__device__ __forceinline__ bool uneven_shape_ok(blast_solve_instance_t const& d) {
+    int64_t const n = d.a.rows;
+    bool ok = n >= 0 && n <= 16 * kUnevenNT && d.a.cols == n && d.l.rows == n && d.l.cols == n && d.a.data &&
+              d.l.data;
+    if (d.b.data || d.x.data)
+        ok = ok && d.b.data && d.x.data && d.b.rows == n && d.x.rows == n && d.x.cols == d.b.cols && d.b.cols >= 0 &&
+             d.b.cols <= kOwnRhs;
+    return ok;
+}
+
+__global__ __launch_bounds__(64 * kUnevenNW, 2) void posv_uneven_kernel(const blast_solve_instance_t* inst,
+                                                                        int64_t count, int32_t* queue,
+                                                                        int32_t* info) {
+    __shared__ OwnerLds<kUnevenNT> lds;
+    __shared__ int sIdx;
+    for (;;) {
+        if (threadIdx.x == 0) sIdx = atomicAdd(&queue[0], 1);
+        __syncthreads();
+        int const q = sIdx;
+        __syncthreads();
+        if (q >= count) break;  // every workgroup reaches this exit
+        int const idx = queue[1 + q];
+        if (idx < 0 || idx >= count) continue;
+        blast_solve_instance_t const d = inst[idx];
+        int32_t* const inf = info ? info + idx : nullptr;
+        if (!uneven_shape_ok(d)) {
+            if (inf && threadIdx.x == 0) *inf = -1;
+            continue;
+        }
+        Mat<double> const A = Mat<double>::from(d.a), L = Mat<double>::from(d.l);
+        Mat<double> const B = Mat<double>::from(d.b), X = Mat<double>::from(d.x);
+        int const n = int(d.a.rows), nrhs = d.b.data ? int(d.b.cols) : 0;
+        switch ((n + 15) / 16) {
+            case 0:
+            case 1: chol_owner_body<1, kUnevenNW, false>(A, L, B, X, n, nrhs, 0, inf, lds); break;
+            case 2: chol_owner_body<2, kUnevenNW, false>(A, L, B, X, n, nrhs, 0, inf, lds); break;
+            case 3: chol_owner_body<3, kUnevenNW, false>(A, L, B, X, n, nrhs, 0, inf, lds); break;
+            case 4: chol_owner_body<4, kUnevenNW, false>(A, L, B, X, n, nrhs, 0, inf, lds); break;
+            case 5: chol_owner_body<5, kUnevenNW, false>(A, L, B, X, n, nrhs, 0, inf, lds); break;
+            case 6: chol_owner_body<6, kUnevenNW, false>(A, L, B, X, n, nrhs, 0, inf, lds); break;
+            case 7: chol_owner_body<7, kUnevenNW, false>(A, L, B, X, n, nrhs, 0, inf, lds); break;
+            case 8: chol_owner_body<8, kUnevenNW, false>(A, L, B, X, n, nrhs, 0, inf, lds); break;
+            case 9: chol_owner_body<9, kUnevenNW, false>(A, L, B, X, n, nrhs, 0, inf, lds); break;
+            case 10: chol_owner_body<10, kUnevenNW, false>(A, L, B, X, n, nrhs, 0, inf, lds); break;
+            case 11: chol_owner_body<11, kUnevenNW, false>(A, L, B, X, n, nrhs, 0, inf, lds); break;
+            default: chol_owner_body<12, kUnevenNW, false>(A, L, B, X, n, nrhs, 0, inf, lds); break;
+        }
+    }
+}
+
+__global__ void zero_queue_counter_kernel(int32_t* p) { *p = 0; }
+
 template <int NT, int NW, int WPE = 1>
 int launch_owner_nt(hipStream_t s, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int n, int nrhs,
                     int64_t batch, int32_t* info, const int32_t* only_if) {
@@ -245,5 +326,21 @@ int launch_chol_owner(hipStream_t s, Mat<double> A, Mat<double> L, Mat<double> B
         default: return launch_owner_nt<16, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
     }
 }
+
+// Uneven batched Cholesky (+ solve): queue counter reset by a kernel (no
+// memset, capturable), then one persistent launch of at most `grid`
+// workgroups (0: min(count, 2048)).
+int launch_posv_uneven(hipStream_t s, const blast_solve_instance_t* inst, int64_t count, int32_t* queue,
+                       int32_t* info, int grid) {
+    if (count == 0) return 0;
+    hipLaunchKernelGGL(zero_queue_counter_kernel, dim3(1), dim3(1), 0, s, queue);
+    int r = check_launch("zero_queue_counter_kernel");
+    if (r) return r;
+    int64_t const g = grid > 0 ? (grid < count ? grid : count) : (count < 2048 ? count : 2048);
+    hipLaunchKernelGGL(posv_uneven_kernel, dim3(unsigned(g)), dim3(64 * kUnevenNW), 0, s, inst, count, queue, info);
+    return check_launch("posv_uneven_kernel");
+}
+
+int uneven_max_n() { return 16 * kUnevenNT; }
 
 }  // namespace blast_hip

@@ -25,7 +25,7 @@ struct Mat {
     int32_t rows, cols, spacing;
     int32_t panel, trans;
 
-    __host__ static Mat from(const blast_matrix_t& m) {
+    __host__ __device__ static Mat from(const blast_matrix_t& m) {
         Mat r;
         r.data = static_cast<T*>(m.data);
         r.stride = m.batch_stride;

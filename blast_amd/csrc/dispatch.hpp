@@ -62,6 +62,11 @@ int launch_chol_owner(hipStream_t, Mat<double> A, Mat<double> L, Mat<double> B, 
 // when non-null, only instances b with only_if[b] < 0 are processed.
 int launch_chol_team(hipStream_t, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int64_t batch,
                      int32_t* info, const int32_t* only_if = nullptr);
+// chol_owner.hip: uneven batch, one persistent launch (queue: count + 1
+// int32, entries 1..count the order); n <= uneven_max_n().
+int launch_posv_uneven(hipStream_t, const blast_solve_instance_t* inst, int64_t count, int32_t* queue,
+                       int32_t* info, int grid);
+int uneven_max_n();
 // sposv_ir.hip: returns 1 when the shape is not covered
 int launch_sposv_ir(hipStream_t, int bf16, Mat<double> A, Mat<double> B, Mat<double> X, int itmax, double tol,
                     int64_t batch, int32_t* info, int32_t* iter);

@@ -284,6 +284,36 @@ int blast_hip_dsgesv_noalloc_batched(void* stream, int precision, int itmax, dou
                                      int64_t* ipiv, int64_t ipiv_stride, const blast_matrix_t* B,
                                      const blast_matrix_t* X, int64_t batch, int32_t* info, int32_t* iter);
 
+/* ---- Uneven batched Cholesky solve (FP64) ----
+ * Instances of different sizes in ONE persistent launch: potrf
+ * (dense/Potrf.hpp:66-106, panel/Potrf.hpp:59-106), and when b/x are given
+ * the two triangular solves (dense/Trsm.hpp:17-66), per instance as
+ * blast_hip_dposv_batched.  Each instance brings its own descriptors (any
+ * storage; batch_stride is not used).  b.data = x.data = NULL: potrf only.
+ * L may alias A and X may alias B. */
+typedef struct blast_solve_instance_t {
+    blast_matrix_t a, l, b, x;
+} blast_solve_instance_t;
+
+/* Largest n the uneven kernels take (192). */
+int blast_hip_uneven_max_n(void);
+
+/* Host-side queue order: instance indices sorted by 16-row block count
+ * ceil(n/16) descending (largest first, so that the small instances fill
+ * the tail of the launch), stable within a block count. */
+int blast_hip_uneven_order(int64_t count, const int32_t* n, int32_t* order);
+
+/* inst: DEVICE array of count descriptors.  queue: DEVICE int32 array of
+ * count + 1 entries, entries 1..count a permutation of 0..count-1 (e.g.
+ * blast_hip_uneven_order's); entry 0 is the work counter, reset by a kernel
+ * at the start of every call (no memset: capturable in a hipGraph).  info
+ * (optional DEVICE int32[count]): 0, k > 0 the first pivot column that is
+ * not positive, -1 an instance whose descriptors are inconsistent or whose
+ * n exceeds blast_hip_uneven_max_n() (it is skipped).  Calls that share
+ * `queue` must not overlap. */
+int blast_hip_dposv_uneven_batched(void* stream, const blast_solve_instance_t* inst, int64_t count, int32_t* queue,
+                                   int32_t* info);
+
 /* ---- Uneven batched Riccati recursion (LQR / MPC backward pass), FP64 ----
  * No BLAST function is replaced: this composite strings together the entry
  * points above (gemm -> syrkLower -> potrf -> trsm per step) the way the
