@@ -506,6 +506,37 @@ def adjoint_workload(steps, warmup, batch=4096, nmin=32, nmax=192, seed=20240611
                      "ms_per_step_graph": t_graph * 1e3, "ms_per_step_eager": t_eager * 1e3}
         if kind == "fp64_forward":
             res[kind]["fp64_tflops_graph"] = flops / t_graph / 1e12
+            # the same forward solves as ONE persistent launch over all sizes
+            # (blast_hip_dposv_uneven_batched, size-ordered queue)
+            inst = (capi.SolveInstance * batch)()
+            nlist, k = [], 0
+            for ent in groups:
+                o = ent["float64"]
+                la, lb = o["A"].layout, o["B"].layout
+                for i in range(ent["count"]):
+                    inst[k].a = la.descriptor(o["A"].tensor.data_ptr() + 8 * i * la.batch_stride)
+                    inst[k].l = la.descriptor(o["L"].tensor.data_ptr() + 8 * i * la.batch_stride)
+                    inst[k].b = lb.descriptor(o["B"].tensor.data_ptr() + 8 * i * lb.batch_stride)
+                    inst[k].x = lb.descriptor(o["X"].tensor.data_ptr() + 8 * i * lb.batch_stride)
+                    nlist.append(ent["n"])
+                    k += 1
+            dinst = torch.from_numpy(np.frombuffer(bytes(inst), dtype=np.uint8).copy()).cuda()
+            order = np.zeros(batch, dtype=np.int32)
+            capi.check("blast_hip_uneven_order", capi.lib().blast_hip_uneven_order(
+                batch, np.array(nlist, dtype=np.int32).ctypes.data, order.ctypes.data))
+            queue = torch.from_numpy(np.concatenate([[0], order]).astype(np.int32)).cuda()
+            uinfo = torch.zeros(batch, dtype=torch.int32, device="cuda")
+
+            def one_launch():
+                capi.check("blast_hip_dposv_uneven_batched", capi.lib().blast_hip_dposv_uneven_batched(
+                    torch.cuda.current_stream().cuda_stream, dinst.data_ptr(), batch, queue.data_ptr(),
+                    uinfo.data_ptr()))
+
+            t_one = time_steps(one_launch, steps, warmup)
+            assert int(uinfo.abs().sum()) == 0
+            res["fp64_forward_uneven_one_launch"] = {
+                "instances_per_s": batch / t_one, "ms_per_step": t_one * 1e3, "fp64_tflops": flops / t_one / 1e12,
+                "kernel": "posv_uneven_kernel (one persistent launch, queue ordered by block count)"}
         # check a sample of every 16th size against the FP64 reference adjoint
         if kind != "fp64_forward":
             worst = 0.0
