@@ -77,7 +77,8 @@ int dispatch_dsposv(hipStream_t, int bf16, int itmax, double tol, Mat<double> A,
 // convert.hip: D := S between any two storage layouts (same logical size)
 template <typename T> int launch_convert(hipStream_t, Mat<T> S, Mat<T> D, int64_t batch);
 
-// wg_solve.hip: one workgroup per instance, square n <= 256, nrhs <= 8.
+// wg_solve.hip (mixed precision) and lu_wg.hip (FP64 LU): one workgroup per
+// instance, square n <= 256, nrhs <= 8.
 bool wg_covers(int n, int nrhs);
 int64_t wg_work_elems(int n);  // floats per instance of the mixed-precision factor
 int launch_wg_gesv(hipStream_t, Mat<double> A, int64_t* ipiv, int64_t ipiv_stride, Mat<double> B, Mat<double> X,

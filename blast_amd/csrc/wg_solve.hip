@@ -1,8 +1,8 @@
-// Batched factor + solve for n <= 256, nrhs <= 8: LU with partial pivoting
-// (getrf + getrs) in FP64, and Cholesky (potrf + two trsm) or LU in FP32
-// (optionally BF16 products in the trailing updates) with FP64 iterative
-// refinement (LAPACK dgesv / dsposv / dsgesv contracts).  The FP64 Cholesky
-// has its own register-resident kernels (posv_tiles.hip, chol_owner.hip).
+// Batched mixed-precision factor + solve for n <= 256, nrhs <= 8: Cholesky
+// (potrf + two trsm) or LU with partial pivoting in FP32 (optionally BF16
+// products in the trailing updates) with FP64 iterative refinement (LAPACK
+// dsposv / dsgesv contracts).  The FP64 factorisations have their own
+// kernels (posv_tiles.hip and chol_owner.hip for Cholesky, lu_wg.hip for LU).
 //
 // One 256-thread workgroup (4 wavefronts) per instance.  The factor F lives
 // in global memory (the caller's L / A buffer in FP64, a dense work buffer in
@@ -600,13 +600,6 @@ int64_t wg_work_elems(int n) {
 }
 
 bool wg_covers(int n, int nrhs) { return n >= 1 && n <= 16 * kWgMaxNT && nrhs >= 0 && nrhs <= kWgRhs; }
-
-// FP64 LU with partial pivoting in place of A, pivots to ipiv, solve into X.
-int launch_wg_gesv(hipStream_t s, Mat<double> A, int64_t* ipiv, int64_t ipiv_stride, Mat<double> B, Mat<double> X,
-                   int nrhs, int64_t batch, int32_t* info, const int32_t* only_if) {
-    return launch_wg<double, true, false>(
-        s, make_args(A, B, X, nrhs, 0, 0.0, info, nullptr, only_if, ipiv, ipiv_stride), A, batch);
-}
 
 // Mixed precision: factor in FP32 (bf16: BF16 products in the trailing
 // updates) into `work` (wg_work_elems(n) floats per instance), refine in FP64.
