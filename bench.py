@@ -604,6 +604,12 @@ def main():
 
     from blast_amd import capi
 
+    if args.calibrate_copy:
+        src = torch.ones(256 * 2 ** 20 // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if args.only in ("gemm", "riccati", "mixed256", "adjoint", "convert"):
         res = {"gemm": lambda: gemm_workload(args.steps, args.warmup),
                "riccati": lambda: riccati_workload(args.steps, args.warmup, args.riccati_count),
@@ -613,12 +619,6 @@ def main():
         print(json.dumps({args.only: res}))
         return
     batch = args.batch
-    if args.calibrate_copy:
-        src = torch.ones(256 * 2 ** 20 // 8, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
     la, lb, abuf, bbuf = make_inputs(batch)
     lbuf = torch.zeros_like(abuf)
     xbuf = torch.zeros_like(bbuf)
