@@ -685,10 +685,10 @@ def main():
     # roofline of the kernel behind `value`; the plain FP64 kernel's beside it
     t_mixed = mixed["ms_per_step"] / 1e3
     roof_plain = {
-        "bound": "hbm", "kernel": "posv_wave_kernel<8>",
+        "bound": "hbm", "kernel": "posv_ll_kernel<8>",
         "achieved": BYTES_PLAIN * batch / t_plain / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
         "frac": BYTES_PLAIN * batch / t_plain / 1e9 / PEAK_HBM_GBS,
-        "traffic": profiled_traffic("posv_wave_kernel<8>"), "algorithmic_bytes_per_instance": BYTES_PLAIN,
+        "traffic": profiled_traffic("posv_ll_kernel<8>"), "algorithmic_bytes_per_instance": BYTES_PLAIN,
         "fp64_tflops": FLOPS_PER_INSTANCE * batch / t_plain / 1e12,
         "fp64_frac_of_peak": FLOPS_PER_INSTANCE * batch / t_plain / 1e12 / PEAK_FP64_TFLOPS,
     }

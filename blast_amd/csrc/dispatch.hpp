@@ -44,7 +44,7 @@ template <typename T>
 int getrs_backward(hipStream_t, Mat<T> LU, const int64_t* ipiv, int64_t ipiv_stride, Mat<T> X, Mat<T> Xb, Mat<T> Ab,
                    Mat<T> Bb, int64_t batch);
 
-// posv_tiles.hip: one wavefront per instance, n = 16 NT <= 128, A and L in
+// posv_ll.hip: one wavefront per instance (two per SIMD), n = 16 NT <= 128, A and L in
 // panel storage, nrhs <= 4; returns 1 when the shape is not covered.  only_if
 // as launch_chol_team.
 int launch_posv_tiles(hipStream_t, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int64_t batch,

@@ -29,6 +29,19 @@ __device__ __forceinline__ void static_for(F&& f) {
     }
 }
 
+// Returns v unchanged, opaque to the optimiser: lane-dependent values
+// (addresses, select masks) are recomputed where they are used instead of
+// being hoisted and held in registers (or spilled) across a whole kernel.
+__device__ __forceinline__ int opaque_v(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+// The same for wavefront-uniform values (tile offsets in scalar registers).
+__device__ __forceinline__ int opaque_s(int v) {
+    asm volatile("" : "+s"(v));
+    return v;
+}
+
 // Operand layouts (both precisions, lane l): A operand A[l % 16][l / 16],
 // B operand B[l / 16][l % 16] for the K-chunk.  Accumulator register r of
 // lane l holds D[row][l % 16]:
@@ -261,30 +274,100 @@ __device__ __forceinline__ T rows_sum(T x) {
     return swap_add<false>(swap_add<true>(x));
 }
 
-// Factorisation of the 16x16 diagonal tile in sD, and its inverse, by one
-// wavefront.  Lane (g, i) = (lane / 16, lane % 16) holds tile row i in the
-// columns it owns, c = 4u + g (u = 0..3): the four 16-lane rows split the
-// columns, and every broadcast of a tile row is a DPP row_newbcast.
-// 1. Cholesky, left-looking: for column k, each row-group forms its share of
-//    a_ik - <l_i, l_k> over its own columns j < k (l_kj broadcast from lane
-//    k), the shares are summed across the groups (rows_sum), then
-//    r = 1/sqrt(d), l_kk = d*r, l_ik = (.)*r, kept by the owner of column k.
-//    The pivot arithmetic (reciprocal square root, scaling) is FP64 for FP32
-//    tiles too; l_ik is rounded to FP32 afterwards.
-// 2. W = L_kk^{-1}, lane (g, i) computing W[i][c] for its columns: row j is
-//    finalised at step j and broadcast from lane j to the lanes below it.
-//    The multipliers -l_ij (strict lower part) are parked in sW first and
-//    scaled by 1/l_jj, broadcast from lane j, as they are used; no per-pivot
-//    lane masks are kept, so the body needs few registers (about 30 VGPRs and
-//    24 SGPRs inlined).
-// On exit sD holds L_kk (zeros above the diagonal) and sW holds W, both as
-// T[m][k] at k*17 + m; with sW == sD the tile holds W.  Returns 0 or the
+// Factorisation of a 16x16 diagonal tile and its inverse by one wavefront,
+// in two phases that work on registers.  Lane (g, i) = (lane / 16, lane % 16)
+// holds tile row i in the columns c = 4u + g (u = 0..3): the four 16-lane rows
+// split the columns, and every broadcast of a tile row is a DPP row_newbcast.
+// This is the accumulator layout of the transposed tile (register u of lane l
+// is T[l % 16][4u + l / 16]), i.e. the MFMA A-operand layout of T itself.
+//
+// chol16_regs, Cholesky, left-looking: for column k, each row-group forms its
+// share of a_ik - <l_i, l_k> over its own columns j < k (l_kj broadcast from
+// lane k), the shares are summed across the groups (rows_sum), then
+// r = 1/sqrt(d), l_kk = d*r, l_ik = (.)*r, kept by the owner of column k.  The
+// pivot arithmetic (reciprocal square root, scaling) is FP64 for FP32 tiles
+// too; l_ik is rounded to FP32 afterwards.  On exit t[u] holds L in the
+// columns c <= i (the other entries are never used), and every lane of row i
+// holds lii = l_ii and rii = 1/sqrt(d_i).  Rows i < k hold values that are
+// never used.  hook(integral_constant<int, s>) runs after step s = 0..15.
+template <typename T, typename Hook>
+__device__ __forceinline__ void chol16_regs(T (&t)[4], int lane, T& lii, double& rii, Hook&& hook) {
+    int const i = lane & 15, g = lane >> 4;
+    lii = T(0);
+    rii = 0.0;
+    static_for<0, 16>([&](auto K) __attribute__((always_inline)) {
+        constexpr int k = decltype(K)::value, gk = k & 3, uk = k >> 2;
+        T p0 = (g == gk) ? t[uk] : T(0), p1 = T(0);
+        static_for<0, 4>([&](auto U) __attribute__((always_inline)) {
+            constexpr int u = decltype(U)::value;
+            if constexpr (4 * u < k) {
+                T& p = u % 2 == 0 ? p0 : p1;
+                T const pr = fma_t(-t[u], row_bcast<k>(t[u]), p);
+                // column 4u + g is finished (j < k) in every group when
+                // 4u + 3 < k; otherwise only where 4u + g < k, the others
+                // holding the diagonal and upper part of rows i and k.  The
+                // product is dropped by a select, not multiplied by 0: a NaN
+                // in the upper part (a full-tile update mirrors the lower one
+                // there) must not reach an earlier pivot.
+                p = (4 * u + 3 < k || 4 * u + g < k) ? pr : p;
+            }
+        });
+        T const v = rows_sum(k > 4 ? p0 + p1 : p0);
+        T const d = row_bcast<k>(v);  // equals v in lane k
+        double const r = rsqrt_nr(double(d));
+        T const lkk = T(double(v) * r);
+        t[uk] = (g == gk) ? lkk : t[uk];
+        lii = (i == k) ? lkk : lii;
+        rii = (i == k) ? r : rii;
+        hook(K);
+    });
+}
+
+// tri_inv16_regs: W = L^{-1} of the 16x16 lower-triangular tile whose
+// multipliers are read by mult(j) = -l_ij (row i = lane % 16; 0 for j >= i),
+// with myr = 1/l_ii of the lane's row.  Lane (g, i) computes W[i][c] for its
+// columns c = 4u + g, kept as z_i[c] = acc_i[c] - delta_ic, where acc_i[c] is
+// the sum of l_im W[m][c] over the rows m < i applied so far; once all are
+// in, W[i][c] = -z_i[c] r_i.  At step j row j is final and rows i > j apply
+// it: z_i += l_ij W[j] = (-l_ij r_j) z_j, one FMA per column with z_j
+// broadcast from lane j.  z_i[c] = 0 for c > i throughout.  On exit w[u] =
+// W[i][4u + g] (0 above the diagonal).  hook(integral_constant<int, 16 + j>)
+// runs after step j = 0..15.
+template <typename T, typename Mult, typename Hook>
+__device__ __forceinline__ void tri_inv16_regs(T (&w)[4], int lane, T myr, Mult&& mult, Hook&& hook) {
+    int const i = lane & 15, g = lane >> 4;
+    T z[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) z[u] = (4 * u + g == i) ? T(-1) : T(0);
+    static_for<0, 16>([&](auto J) __attribute__((always_inline)) {
+        constexpr int j = decltype(J)::value;
+        T const f = mult(j) * row_bcast<j>(myr);  // -l_ij r_j, 0 for i <= j
+        static_for<0, 4>([&](auto U) __attribute__((always_inline)) {
+            constexpr int u = decltype(U)::value;
+            if constexpr (4 * u <= j) z[u] = fma_t(f, row_bcast<j>(z[u]), z[u]);
+        });
+        hook(std::integral_constant<int, 16 + j>{});
+    });
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = (4 * u + g <= i) ? -z[u] * myr : T(0);
+}
+
 // 1-based tile column of the first pivot that is not positive (d <= 0 or NaN
-// makes l_kk NaN), the same value in every lane.  Branch-free: every lane
-// runs the same instructions; all 64 lanes must be active.
-// hook(integral_constant<int, s>) runs after step s = 0..31 of the
-// factorisation (s < 16: column s of L_kk; s >= 16: row s - 16 of W), so that
-// a caller can interleave independent work (MFMA updates) with the chain.
+// makes l_ii NaN), 0 if none; the same value in every lane.
+template <typename T>
+__device__ __forceinline__ int first_bad_pivot(T lii, int lane) {
+    unsigned long long const badl = __ballot(lane < 16 && !(double(lii) > 0.0));
+    return badl ? __builtin_ctzll(badl) + 1 : 0;
+}
+
+// The two phases on a tile in LDS: sD holds the tile (T[m][k] at k*17 + m);
+// on exit sD holds L_kk (zeros above the diagonal) and sW holds W, same
+// layout; with sW == sD the tile holds W.  The multipliers -l_ij are parked in
+// sW between the phases.  Returns first_bad_pivot.  Branch-free: every lane
+// runs the same instructions; all 64 lanes must be active.  hook(s) runs
+// after step s = 0..31 (s < 16: column s of L_kk; s >= 16: row s - 16 of W),
+// so that a caller can interleave independent work (MFMA updates) with the
+// chain; the body alone needs about 30 VGPRs and 24 SGPRs inlined.
 struct NoHook {
     template <typename S>
     __device__ __forceinline__ void operator()(S) const {}
@@ -300,31 +383,9 @@ __device__ __forceinline__ int diag_factor_hooked(T* sD_generic, T* sW_generic, 
     T t[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) t[u] = sD[(4 * u + g) * kTileStride + i];
-    static_for<0, 16>([&](auto K) __attribute__((always_inline)) {
-        constexpr int k = decltype(K)::value, gk = k & 3, uk = k >> 2;
-        T p0 = (g == gk) ? t[uk] : T(0), p1 = T(0);
-        static_for<0, 4>([&](auto U) __attribute__((always_inline)) {
-            constexpr int u = decltype(U)::value;
-            if constexpr (4 * u < k) {
-                T& p = u % 2 == 0 ? p0 : p1;
-                T const pr = fma_t(-t[u], row_bcast<k>(t[u]), p);
-                // column 4u + g is finished (j < k) in every group when
-                // 4u + 3 < k; otherwise only where 4u + g < k, the others
-                // holding the diagonal and upper part of rows i and k.  The
-                // product is dropped by a select, not multiplied by 0: a NaN
-                // in the upper part (the full-tile trailing update mirrors
-                // the lower one there) must not reach an earlier pivot.
-                p = (4 * u + 3 < k || 4 * u + g < k) ? pr : p;
-            }
-        });
-        T const v = rows_sum(k > 4 ? p0 + p1 : p0);
-        T const d = row_bcast<k>(v);  // equals v in lane k
-        double const r = rsqrt_nr(double(d));
-        // rows i < k hold values that are never used (the writes below mask
-        // them; rows_sum and the broadcasts of rows >= k never read them)
-        t[uk] = (g == gk) ? T(double(v) * r) : t[uk];
-        hook(K);
-    });
+    T lii_unused;
+    double rii_unused;
+    chol16_regs(t, lane, lii_unused, rii_unused, hook);
     // (sW may be sD: L_kk is written and its diagonal read back before the
     // multipliers overwrite it)
 #pragma unroll
@@ -338,29 +399,11 @@ __device__ __forceinline__ int diag_factor_hooked(T* sD_generic, T* sW_generic, 
         int const c = 4 * u + g;
         sW[c * kTileStride + i] = (c < i) ? -t[u] : T(0);
     }
-    T const myr = T(1.0 / double(lii));
-    // W row by row, kept as z_i[c] = acc_i[c] - delta_ic, where acc_i[c] is
-    // the sum of l_im W[m][c] over the rows m < i applied so far; once all are
-    // in, W[i][c] = -z_i[c] r_i (r_i = 1/l_ii).  At step j row j is final and
-    // rows i > j apply it: z_i += l_ij W[j] = (-l_ij r_j) z_j, one FMA per
-    // column with z_j broadcast from lane j.  z_i[c] = 0 for c > i throughout.
-    T z[4];
+    T w[4];
+    tri_inv16_regs(w, lane, T(1.0 / double(lii)), [&](int j) { return sW[j * kTileStride + i]; }, hook);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) z[u] = (4 * u + g == i) ? T(-1) : T(0);
-    static_for<0, 16>([&](auto J) __attribute__((always_inline)) {
-        constexpr int j = decltype(J)::value;
-        T const f = sW[j * kTileStride + i] * row_bcast<j>(myr);  // -l_ij r_j, 0 for i <= j
-        static_for<0, 4>([&](auto U) __attribute__((always_inline)) {
-            constexpr int u = decltype(U)::value;
-            if constexpr (4 * u <= j) z[u] = fma_t(f, row_bcast<j>(z[u]), z[u]);
-        });
-        hook(std::integral_constant<int, 16 + j>{});
-    });
-#pragma unroll
-    for (int u = 0; u < 4; ++u) sW[(4 * u + g) * kTileStride + i] = (4 * u + g <= i) ? -z[u] * myr : T(0);
-    // first pivot that is not positive: lanes 0..15 hold l_00..l_15,15
-    unsigned long long const badl = __ballot(lane < 16 && !(double(lii) > 0.0));
-    return badl ? __builtin_ctzll(badl) + 1 : 0;
+    for (int u = 0; u < 4; ++u) sW[(4 * u + g) * kTileStride + i] = w[u];
+    return first_bad_pivot(lii, lane);
 }
 
 template <typename T>

@@ -21,7 +21,7 @@ LIB = os.path.join(ROOT, "blast_amd", "libblast_hip.so")
 
 # family (substring of the mangled name) -> limits
 RULES = {
-    "posv_wave_kernel": dict(scratch=0, vgpr_spill=0, sgpr_spill=0, calls=False),
+    "posv_ll_kernel": dict(scratch=0, vgpr_spill=0, sgpr_spill=0, calls=False),
     "chol_team_kernel": dict(scratch=0, vgpr_spill=0, calls=False),
     "chol_owner_kernel": dict(scratch=0, vgpr_spill=0, calls=False),
     "wg_solve_kernel": dict(scratch=0, vgpr_spill=0, calls=False),
