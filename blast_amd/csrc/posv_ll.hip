@@ -130,14 +130,21 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
             t[tri(i, j)] = load_t(pa, sa, loa, i, j);
         });
     };
-    if (nrhs > 0) {
+    load_col(std::integral_constant<int, 0>{});
+    if (nrhs > 0) {  // every load issued before the first LDS store
         Ref<double> bb(B, b);
-        for (int e = lane; e < NR; e += 64) {
-            int const row = e / kMaxRhs, c = e % kMaxRhs;
-            sm.sW[row >> 4][ys(row & 15, c)] = c < nrhs ? bb(row, c) : 0.0;
+        double v[NR / 64];
+#pragma unroll
+        for (int e = 0; e < NR / 64; ++e) {
+            int const idx = lane + 64 * e, row = idx / kMaxRhs, c = idx % kMaxRhs;
+            v[e] = c < nrhs ? bb(row, c) : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < NR / 64; ++e) {
+            int const idx = lane + 64 * e, row = idx / kMaxRhs, c = idx % kMaxRhs;
+            sm.sW[row >> 4][ys(row & 15, c)] = v[e];
         }
     }
-    load_col(std::integral_constant<int, 0>{});
 
     // Right-hand-side blocks (16 rows x kMaxRhs columns) live in LDS and take
     // part in v_mfma_f64_4x4x4f64 products: four 4x4x4 blocks b = 0..3, A
@@ -154,11 +161,31 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
     int const yd = ys(4 * (m >> 2) + g, m & (kMaxRhs - 1));
     auto mma4 = [](double a, double bv, double c) { return __builtin_amdgcn_mfma_f64_4x4x4f64(a, bv, c, 0, 0, 0); };
     int inf = 0;
+    // factor tiles L_kb,j in the accumulator layout, read back for the
+    // backward solve (row kb, j < kb)
+    d4 lt[NTILES];
+    auto load_row = [&](auto KB) __attribute__((always_inline)) {
+        constexpr int kb = decltype(KB)::value;
+        static_for<0, kb>([&](auto J) __attribute__((always_inline)) {
+            constexpr int j = decltype(J)::value;
+            lt[tri(kb, j)] = load_acc(pl, sl, kb, j, lane);
+        });
+    };
 
     static_for<0, NT>([&](auto J) __attribute__((always_inline)) {
         constexpr int j = decltype(J)::value;
         using Ops = StepOps<NT, j>;
         double* const sw = sm.sW[j];
+        if constexpr (j == NT - 1 && NT > 1) {
+            // every off-diagonal factor tile is stored: the backward solve's
+            // read-back of the last row starts here, after a barrier (a
+            // workgroup-scope release and acquire; the workgroup is this one
+            // wavefront) that makes the stores visible to the other lanes
+            if (nrhs > 0) {
+                __syncthreads();
+                load_row(std::integral_constant<int, NT - 1>{});
+            }
+        }
         // diagonal tile: A_jj -= sum_k L_jk L_jk^T (whole tile, symmetric)
         static_for<0, j>([&](auto K) __attribute__((always_inline)) {
             constexpr int k = decltype(K)::value;
@@ -168,24 +195,36 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
         // forward: r_j = b_j - sum_k L_jk y_k (result layout)
         double r = 0.0;
         if (nrhs > 0) r = sw[yd];
-        // the diagonal factorisation, with the column's other updates and
-        // the forward update spread over its 16 steps
-        double dt[4] = {t[tri(j, j)][0], t[tri(j, j)][1], t[tri(j, j)][2], t[tri(j, j)][3]};
-        double lii_unused, rii_unused;
-        chol16_regs(dt, opaque_v(lane), lii_unused, rii_unused, [&](auto S) __attribute__((always_inline)) {
+        // the diagonal factorisation (one tile row per lane, through LDS:
+        // b_j has been read), with the column's other updates and the
+        // forward update spread over its 16 steps
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sw[(4 * q + g) * kTileStride + m] = t[tri(j, j)][q];
+        double row[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) row[c] = sw[c * kTileStride + m];
+        chol16_rows(row, [&](auto S) __attribute__((always_inline)) {
             constexpr int s = decltype(S)::value;
             static_for<Ops::ops * s / 16, Ops::ops * (s + 1) / 16>([&](auto O) __attribute__((always_inline)) {
                 constexpr int e = Ops::encode(decltype(O)::value);
                 constexpr int k = e >> 16, q = (e >> 8) & 255, i = e & 255;
                 if constexpr (i < NT)
                     t[tri(i, j)] = M::mma(-t[tri(j, k)][q], t[tri(i, k)][q], t[tri(i, j)]);
-                else if (nrhs > 0)
+                else  // (without a branch on nrhs: r is not used when nrhs = 0)
                     r = mma4(-t[tri(j, k)][q], rhs_b(k, q), r);
             });
         });
+        if (g == 0)
+#pragma unroll
+            for (int c = 0; c < 16; ++c) sw[c * kTileStride + m] = row[c];
+        double dt[4];  // L_jj in the transposed layout (L[m][4u + g])
+#pragma unroll
+        for (int u = 0; u < 4; ++u) dt[u] = sw[(4 * u + g) * kTileStride + m];
         // the next column's tiles, while this one is inverted and applied
         __builtin_amdgcn_sched_barrier(0);
         if constexpr (j + 1 < NT) load_col(std::integral_constant<int, j + 1>{});
+        if constexpr (j == NT - 1 && NT > 2)
+            if (nrhs > 0) load_row(std::integral_constant<int, NT - 2>{});
         // L_jj: lower part; zeros in the strict upper part of each 4x4
         // diagonal block (panel storage)
         {
@@ -208,7 +247,10 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
         int const bad = first_bad_pivot(lii, lane);
         if (bad && inf == 0) inf = 16 * j + bad;
         double w[4];
-        tri_inv16_regs(w, opaque_v(lane), 1.0 / lii, [&](int jj) { return sw[jj * kTileStride + m]; }, NoHook{});
+        double rc = __builtin_amdgcn_rcp(lii);  // 1 / l_ii: estimate and two Newton steps
+        rc = fma_t(rc, fma_t(-lii, rc, 1.0), rc);
+        rc = fma_t(rc, fma_t(-lii, rc, 1.0), rc);
+        tri_inv16_regs(w, opaque_v(lane), rc, [&](int jj) { return sw[jj * kTileStride + m]; }, NoHook{});
 #pragma unroll
         for (int u = 0; u < 4; ++u)  // lower triangle, for the backward solve
             if (4 * u + g <= m) sw[(4 * u + g) * kTileStride + m] = w[u];
@@ -244,26 +286,11 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
     });
 
     if (nrhs > 0) {
-        // The backward solve reads L's off-diagonal tiles in the accumulator
-        // layout, from other lanes than the ones that stored them: the
-        // barrier (a workgroup-scope release and acquire; the workgroup is
-        // this one wavefront) makes the stores visible first.
-        __syncthreads();
         // right-looking: x_kb = W_kb^T r_kb (r_kb final), then
         // r_j -= L_kb,j^T x_kb for j < kb; each result passes through LDS
         // into the B-operand layout.
         double r[NT];
         static_for<0, NT>([&](auto I) __attribute__((always_inline)) { r[I] = sm.sW[I][yd]; });
-        d4 lt[NTILES];
-        auto load_row = [&](auto KB) __attribute__((always_inline)) {
-            constexpr int kb = decltype(KB)::value;
-            static_for<0, kb>([&](auto J) __attribute__((always_inline)) {
-                constexpr int j = decltype(J)::value;
-                lt[tri(kb, j)] = load_acc(pl, sl, kb, j, lane);
-            });
-        };
-        load_row(std::integral_constant<int, NT - 1>{});
-        if constexpr (NT > 2) load_row(std::integral_constant<int, NT - 2>{});
         static_for<0, NT>([&](auto KK) __attribute__((always_inline)) {
             constexpr int kb = NT - 1 - decltype(KK)::value;
             if constexpr (kb >= 3) load_row(std::integral_constant<int, kb - 2>{});

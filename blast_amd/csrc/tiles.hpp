@@ -221,13 +221,14 @@ __device__ __forceinline__ float row_bcast(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x150 + C, 0xF, 0xF, false));
 }
 
-// 1/sqrt(d): hardware estimate refined by Newton steps (about 1 ulp).
+// 1/sqrt(d): the hardware estimate r refined by one third-order step (about
+// 1 ulp).  With eps = (1 - d r^2) / 2, 1/sqrt(d) = r (1 - 2 eps)^(-1/2)
+// = r (1 + eps + 1.5 eps^2 + O(eps^3)); four dependent operations after the
+// estimate instead of the six of two Newton steps.
 __device__ __forceinline__ double rsqrt_nr(double d) {
-    double r = __builtin_amdgcn_rsq(d);
-    double const h = 0.5 * d;
-    r = fma_t(r, fma_t(-h * r, r, 0.5), r);
-    r = fma_t(r, fma_t(-h * r, r, 0.5), r);
-    return r;
+    double const r = __builtin_amdgcn_rsq(d);
+    double const eps = fma_t(-0.5 * d * r, r, 0.5);
+    return fma_t(r * eps, fma_t(1.5, eps, 1.0), r);
 }
 __device__ __forceinline__ float rsqrt_nr(float d) {
     float r = __builtin_amdgcn_rsqf(d);
@@ -320,6 +321,30 @@ __device__ __forceinline__ void chol16_regs(T (&t)[4], int lane, T& lii, double&
         lii = (i == k) ? lkk : lii;
         rii = (i == k) ? r : rii;
         hook(K);
+    });
+}
+
+// chol16_rows: the same factorisation with one tile row per lane: lane
+// (g, m) holds row m in all 16 columns (the four 16-lane rows hold copies).
+// Right-looking: at step k the pivot a_kk is broadcast from lane k,
+// r = 1/sqrt(a_kk), column k is scaled (l_mk = a_mk r, l_kk = a_kk r) and the
+// columns c > k are updated, a_mc -= l_mk l_ck with l_ck broadcast from lane
+// c.  A step's dependent chain is only the pivot broadcast, the reciprocal
+// square root and the update of a_{k+1,k+1}; the other updates are
+// independent of it.  On exit row[c] = l_mc for c <= m; the entries above
+// the diagonal hold values that are never used (and never reach a pivot).
+// hook(integral_constant<int, k>) runs after column k is scaled.
+template <typename Hook>
+__device__ __forceinline__ void chol16_rows(double (&row)[16], Hook&& hook) {
+    static_for<0, 16>([&](auto K) __attribute__((always_inline)) {
+        constexpr int k = decltype(K)::value;
+        double const r = rsqrt_nr(row_bcast<k>(row[k]));
+        row[k] *= r;
+        hook(K);
+        static_for<k + 1, 16>([&](auto C) __attribute__((always_inline)) {
+            constexpr int c = decltype(C)::value;
+            row[c] = fma_t(-row[k], row_bcast<c>(row[k]), row[c]);
+        });
     });
 }
 
