@@ -87,18 +87,32 @@ __device__ __forceinline__ d4 load_acc(const double* p, int spacing, int ti, int
     return d4{q[0], q[spacing], q[2 * spacing], q[3 * spacing]};
 }
 
-// The operations of step j that do not wait for the diagonal factorisation:
-// for k < j (outer), K-chunk q, then row i = j+1..NT-1 (the off-diagonal
-// update A_ij^T -= L_jk L_ik^T) and i = NT (the forward update
-// r_j -= L_jk y_k), so that consecutive MFMAs accumulate into different
-// tiles.  Encoded as (k << 16) | (q << 8) | i.
+// The operations of step j that do not wait for its diagonal factorisation,
+// issued between the 16 steps of that factorisation (their MFMAs fill the
+// dependency stalls of its VALU chain), in this order:
+//   kind 0: the deferred panel of block column j-1, L_i,j-1^T = W_j-1 A_i,j-1^T
+//           for i = j+1..NT-1 (tile (j, j-1) was formed at the end of step
+//           j-1), four K-chunks per tile;
+//   kind 1: y_j-1 = W_j-1 r_j-1, four K-chunks;
+//   kind 2: for k < j (outer), K-chunk q, then row i = j+1..NT-1 (the update
+//           A_ij^T -= L_jk L_ik^T) and i = NT (the forward update
+//           r_j -= L_jk y_k), so that consecutive MFMAs accumulate into
+//           different tiles.
+// The k = j-1 terms come last, after the panel tiles and y_j-1 they read.
+// Encoded as (kind << 24) | (k << 16) | (q << 8) | i.
 template <int NT, int J>
 struct StepOps {
+    static constexpr int panel = J >= 1 ? 4 * (NT - 1 - J) : 0;
+    static constexpr int yops = J >= 1 ? 4 : 0;
     static constexpr int rows = NT - J;  // NT - 1 - J tiles and the right-hand sides
-    static constexpr int ops = J * 4 * rows;
+    static constexpr int ops = panel + yops + J * 4 * rows;
     static constexpr int encode(int o) {
+        if (o < panel) return (0 << 24) | ((o % 4) << 8) | (J + 1 + o / 4);
+        o -= panel;
+        if (o < yops) return (1 << 24) | (o << 8);
+        o -= yops;
         int const k = o / (4 * rows), q = (o / rows) % 4, i = J + 1 + o % rows;
-        return (k << 16) | (q << 8) | i;
+        return (2 << 24) | (k << 16) | (q << 8) | i;
     }
 };
 
@@ -172,6 +186,15 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
         });
     };
 
+    auto store_l = [&](int i, int jj, d4 v) __attribute__((always_inline)) {  // tile (i, jj), transposed layout
+        double* const q0 = pl + int64_t(opaque_s(4 * i * sl)) + 64 * jj + lol;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) q0[16 * q] = v[q];
+    };
+    double wp[4] = {0.0, 0.0, 0.0, 0.0};  // W of the previous block column
+    d4 pz;                                // deferred panel tile being formed
+    double ypz[2];
+
     static_for<0, NT>([&](auto J) __attribute__((always_inline)) {
         constexpr int j = decltype(J)::value;
         using Ops = StepOps<NT, j>;
@@ -207,11 +230,22 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
             constexpr int s = decltype(S)::value;
             static_for<Ops::ops * s / 16, Ops::ops * (s + 1) / 16>([&](auto O) __attribute__((always_inline)) {
                 constexpr int e = Ops::encode(decltype(O)::value);
-                constexpr int k = e >> 16, q = (e >> 8) & 255, i = e & 255;
-                if constexpr (i < NT)
+                constexpr int kind = e >> 24, k = (e >> 16) & 255, q = (e >> 8) & 255, i = e & 255;
+                if constexpr (kind == 0) {  // deferred panel tile (i, j-1)
+                    pz = M::mma(wp[q], t[tri(i, j - 1)][q], q == 0 ? d4{0.0, 0.0, 0.0, 0.0} : pz);
+                    if constexpr (q == 3) {
+                        t[tri(i, j - 1)] = pz;
+                        store_l(i, j - 1, pz);
+                    }
+                } else if constexpr (kind == 1) {  // y_j-1 = W_j-1 r_j-1 (r_j-1 in LDS)
+                    if constexpr (q < 2) ypz[q] = mma4(wp[q], rhs_b(j - 1, q), 0.0);
+                    else ypz[q - 2] = mma4(wp[q], rhs_b(j - 1, q), ypz[q - 2]);
+                    if constexpr (q == 3) sm.sW[j - 1][yd] = ypz[0] + ypz[1];
+                } else if constexpr (i < NT) {
                     t[tri(i, j)] = M::mma(-t[tri(j, k)][q], t[tri(i, k)][q], t[tri(i, j)]);
-                else  // (without a branch on nrhs: r is not used when nrhs = 0)
+                } else {  // (without a branch on nrhs: r is not used when nrhs = 0)
                     r = mma4(-t[tri(j, k)][q], rhs_b(k, q), r);
+                }
             });
         });
         if (g == 0)
@@ -254,36 +288,27 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
 #pragma unroll
         for (int u = 0; u < 4; ++u)  // lower triangle, for the backward solve
             if (4 * u + g <= m) sw[(4 * u + g) * kTileStride + m] = w[u];
-        // y_j = W_j r_j: r_j passes through LDS into the B-operand layout
-        if (nrhs > 0) {
-            sw[yd] = r;
-            double y = mma4(w[0], rhs_b(j, 0), 0.0);
-            double y1 = mma4(w[1], rhs_b(j, 1), 0.0);
-            y = mma4(w[2], rhs_b(j, 2), y);
-            y1 = mma4(w[3], rhs_b(j, 3), y1);
-            sw[yd] = y + y1;
+        // r_j final: kept in LDS (B-operand layout) for y_j = W_j r_j, which
+        // the next step forms among its hooked operations
+        sw[yd] = r;
+        // the panel tile the next diagonal tile waits for; the others are
+        // formed during the next diagonal factorisation
+        if constexpr (j + 1 < NT) {
+            d4 z = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) z = M::mma(w[q], t[tri(j + 1, j)][q], z);
+            t[tri(j + 1, j)] = z;
+            store_l(j + 1, j, z);
         }
-        // panel: L_ij^T = W_j A_ij^T, two tiles' chains interleaved
-        static_for<0, (NT - j) / 2>([&](auto P) __attribute__((always_inline)) {
-            constexpr int i0 = j + 1 + 2 * decltype(P)::value, i1 = i0 + 1 < NT ? i0 + 1 : i0;
-            d4 z0 = {0.0, 0.0, 0.0, 0.0}, z1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                z0 = M::mma(w[q], t[tri(i0, j)][q], z0);
-                if constexpr (i1 != i0) z1 = M::mma(w[q], t[tri(i1, j)][q], z1);
-            }
-            t[tri(i0, j)] = z0;
-            double* const q0 = pl + int64_t(opaque_s(4 * i0 * sl)) + 64 * j + lol;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) q0[16 * q] = z0[q];
-            if constexpr (i1 != i0) {
-                t[tri(i1, j)] = z1;
-                double* const q1 = pl + int64_t(opaque_s(4 * i1 * sl)) + 64 * j + lol;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) q1[16 * q] = z1[q];
-            }
-        });
+        for (int q = 0; q < 4; ++q) wp[q] = w[q];
     });
+    {  // y of the last block column
+        double ya = mma4(wp[0], rhs_b(NT - 1, 0), 0.0), yb = mma4(wp[1], rhs_b(NT - 1, 1), 0.0);
+        ya = mma4(wp[2], rhs_b(NT - 1, 2), ya);
+        yb = mma4(wp[3], rhs_b(NT - 1, 3), yb);
+        sm.sW[NT - 1][yd] = ya + yb;
+    }
 
     if (nrhs > 0) {
         // right-looking: x_kb = W_kb^T r_kb (r_kb final), then
