@@ -5,9 +5,12 @@
 //
 // One 256-thread workgroup (4 wavefronts) per instance.  The factor lives in
 // global memory (512 KB at n = 256 does not fit one CU) and is updated
-// right-looking by 16-column blocks kb (rows r0 = 16 kb..):
+// right-looking by 32-column block columns kb (rows r0 = 32 kb..), so that
+// each trailing tile makes one round trip through HBM per 32 columns
+// (2.3 MB per instance at n = 256, against 5 MB for 16-column blocks) and
+// the per-block phases run n / 32 times:
 //   panel: the block column's rows r0.. are held in registers, one row of
-//     16 values per thread, and factored there column by column (getf2).
+//     32 values per thread, and factored there column by column (getf2).
 //     Per column each wavefront reduces its (|value|, position) pairs with
 //     DPP and lane swaps and posts its winning row to LDS; after one
 //     barrier every thread combines the four candidates itself.  Rows never
@@ -16,21 +19,22 @@
 //     multipliers and the rank-1 update stay in registers: one barrier per
 //     column.
 //   swaps: applied to the columns right of the panel from the panel's
-//     position table (which row each position received), 16 lanes per
-//     column: rows r0..r0+15 land in LDS as the block row A_kj and the
+//     position table (which row each position received), 32 lanes per
+//     column: rows r0..r0+31 land in LDS as the block row A_kj and the
 //     displaced rows are written back; the columns left of the panel receive
 //     every later block's swaps at the end, in one pass per block, from the
 //     composite row permutation (each thread traces its row back through
 //     the swaps).
-//   U_kj = L_kk^{-1} A_kj and the trailing update A_ij -= L_ik U_kj on MFMA,
-//     both computed transposed (U_kj^T = A_kj^T L_kk^{-T}, A_ij^T -=
-//     U_kj^T L_ik^T) so that each lane's accumulator elements run down a
-//     column of A: coalesced column-major loads and stores.  The trailing
-//     tiles stream through registers eight per wavefront, the next group's
-//     loads issued before the current group's MFMAs.
+//   U_kj = L_kk^{-1} A_kj and the trailing update A_ij -= L_ik U_kj on MFMA
+//     (K = 32: two 16x16 tiles per operand), both computed transposed
+//     (U_kj^T = A_kj^T L_kk^{-T}, A_ij^T -= U_kj^T L_ik^T) so that each
+//     lane's accumulator elements run down a column of A: coalesced
+//     column-major loads and stores.  The trailing tiles stream through
+//     registers eight per wavefront, the next group's loads issued before
+//     the current group's MFMAs.
 // Solve: B's rows permuted once (the composite permutation of all swaps),
-// the diagonal blocks' inverses (unit-lower and upper) formed in parallel,
-// then block substitution on MFMA.
+// the 16x16 diagonal blocks' inverses (unit-lower and upper) formed in
+// parallel, then block substitution on MFMA.
 // Algorithm references: dense/Getrf.hpp:52-119 (blocking, ipiv += k, laswp
 // on both sides, trsm, gemm), dense/Getf2.hpp:52-77 (pivot, swap, division
 // by the pivot, ger), dense/Iamax.hpp:82-95 (first maximum),
@@ -49,6 +53,7 @@ namespace {
 constexpr int kLuThreads = 256;
 constexpr int kLuWaves = kLuThreads / 64;
 constexpr int kLuMaxNT = 16;  // n <= 256: one panel row per thread
+constexpr int kLuBs = 32;     // block column width (two 16x16 tiles)
 constexpr int kLuRhs = 8;     // right-hand-side columns per instance
 constexpr int kLuGroup = 8;   // trailing tiles per wavefront in flight
 
@@ -95,28 +100,44 @@ struct LuArgs {
     int64_t ipiv_stride;
 };
 
+// LDS of one instance: the factor phase's tiles and the solve phase's
+// overlay the same storage.  Tiles are 16x16 with element (row m, col c) at
+// c*17 + m unless noted.
+struct LuFactorLds {
+    double sP[kLuMaxNT][2][kTileElems];      // panel: tile (row tile t, column tile s) of rows r0 + 16 t..
+    double sR[kLuMaxNT - 2][2][kTileElems];  // block row, column tile t, row tile s: A_kj (element (k, m)
+                                             // at k*17 + m), then U_kj^T (accumulator layout)
+    double sW[2][2][kTileElems];             // L_kk^{-1} (unit lower, 32 x 32)
+    double sRow[2][kLuWaves][kLuBs];         // each wavefront's candidate pivot row
+    double sCandV[2][kLuWaves];
+    int sCandI[2][kLuWaves];
+};
+struct LuSolveLds {
+    double sT[3][kLuMaxNT][kTileElems];    // unit-lower inverses, upper inverses, diagonal blocks
+    double sX[kLuMaxNT * 16 * kLuRhs];     // right-hand sides, [row][rhs]
+};
+
 __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs g) {
     using M = Mfma<double>;
-    // Factor phase: sT[0] the panel tiles (tile t: rows r0 + 16 t.., element
-    // (m, c) at c*17 + m), sT[1] the block row (tile t: columns
-    // r0 + 16 (t + 1)..; A_kj with element (k, m) at k*17 + m, then U_kj^T as
-    // an A operand).  Solve phase: sT[0] the unit-lower inverses, sT[1] the
-    // upper inverses, sT[2] the diagonal blocks.
-    __shared__ double sT[3][kLuMaxNT][kTileElems];
-    __shared__ double sWk[kTileElems];                // L_kk^{-1} (unit lower)
-    __shared__ double sRow[2][kLuWaves][16];          // each wavefront's candidate pivot row
-    __shared__ double sCandV[2][kLuWaves];
-    __shared__ int sCandI[2][kLuWaves];
-    __shared__ int sPiv[kLuMaxNT * 16];               // pivot row of every row (absolute)
-    __shared__ int sSrc[kLuMaxNT * 16];               // panel position -> its row before the block's swaps
-    __shared__ double sX[kLuMaxNT * 16 * kLuRhs];     // right-hand sides, [row][rhs]
+    __shared__ union {
+        LuFactorLds f;
+        LuSolveLds s;
+    } sm;
+    __shared__ int sPiv[kLuMaxNT * 16];  // pivot row of every row (absolute)
+    __shared__ int sSrc[kLuMaxNT * 16];  // panel position -> its row before the block's swaps
     __shared__ int sFlag;
+    auto& sP = sm.f.sP;
+    auto& sR = sm.f.sR;
+    auto& sW = sm.f.sW;
+    auto& sRow = sm.f.sRow;
+    auto& sCandV = sm.f.sCandV;
+    auto& sCandI = sm.f.sCandI;
 
     int64_t const b = blockIdx.x;
     if (g.only_if && g.only_if[b] >= 0) return;
     int const tid = threadIdx.x, lane = tid & 63;
     int const w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    int const n = g.n, nrhs = g.nrhs, NT = (n + 15) >> 4, npad = 16 * NT;
+    int const n = g.n, nrhs = g.nrhs, NB = (n + kLuBs - 1) / kLuBs, npad = kLuBs * NB, NT = 2 * NB;
     Ref<double> f(g.A, b);
     // A outside n x n reads as the identity (the factorisation of blkdiag(A, I))
     auto fget = [&](int i, int j) -> double { return (i < n && j < n) ? f(i, j) : (i == j ? 1.0 : 0.0); };
@@ -125,19 +146,19 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs g) {
     };
     if (tid == 0) sFlag = 0;
 
-    for (int kb = 0; kb < NT; ++kb) {
-        int const r0 = 16 * kb, P = npad - r0, mt = NT - kb - 1;
+    for (int kb = 0; kb < NB; ++kb) {
+        int const r0 = kLuBs * kb, P = npad - r0, T0 = 2 * kb + 2, mt = NT - T0;
         // ---- panel: row r0 + tid in registers.  Rows do not move: each
         // thread tracks the position its row holds under the swaps so far
         // (pos < c: a finished row of U), and the pivot search breaks ties by
         // position, as the sequential scan does.
         bool const live = tid < P;
         int pos = tid;
-        double v[16];
+        double v[kLuBs];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) v[c] = live ? fget(r0 + tid, r0 + c) : 0.0;
+        for (int c = 0; c < kLuBs; ++c) v[c] = live ? fget(r0 + tid, r0 + c) : 0.0;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
+        for (int c = 0; c < kLuBs; ++c) {
             int const buf = c & 1;
             bool const cand = live && pos >= c;
             double kv = cand ? iamax_key(v[c], pos - c) : -1.0;
@@ -145,7 +166,7 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs g) {
             wave_argmax(kv, ki);
             if (pos == ki)
 #pragma unroll
-                for (int cc = 0; cc < 16; ++cc) sRow[buf][w][cc] = v[cc];
+                for (int cc = c; cc < kLuBs; ++cc) sRow[buf][w][cc] = v[cc];
             if (lane == 0) sCandV[buf][w] = kv, sCandI[buf][w] = ki;
             __syncthreads();
             double pv = sCandV[buf][0];
@@ -157,96 +178,104 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs g) {
                 wp = pp != before ? q : wp;
             }
             // position c always holds a candidate, so pp is a real position
-            double u[16];
-#pragma unroll
-            for (int cc = 0; cc < 16; ++cc) u[cc] = sRow[buf][wp][cc];
             pos = pos == pp ? c : (pos == c ? pp : pos);
             if (tid == 0) sPiv[r0 + c] = r0 + pp;
-            double const piv = u[c];
+            double const piv = sRow[buf][wp][c];
             if (piv == 0.0) {
                 if (tid == 0 && sFlag == 0 && r0 + c < n) sFlag = r0 + c + 1;
             } else if (live && pos > c) {
                 double const l = v[c] / piv;
                 v[c] = l;
 #pragma unroll
-                for (int cc = c + 1; cc < 16; ++cc) v[cc] = __builtin_fma(-l, u[cc], v[cc]);
+                for (int cc = c + 1; cc < kLuBs; ++cc) v[cc] = __builtin_fma(-l, sRow[buf][wp][cc], v[cc]);
             }
         }
         if (live) {
             sSrc[pos] = tid;
 #pragma unroll
-            for (int cc = 0; cc < 16; ++cc) {
+            for (int cc = 0; cc < kLuBs; ++cc) {
                 fset(r0 + pos, r0 + cc, v[cc]);
-                sT[0][pos >> 4][cc * kTileStride + (pos & 15)] = v[cc];
+                sP[pos >> 4][cc >> 4][(cc & 15) * kTileStride + (pos & 15)] = v[cc];
             }
         }
         __syncthreads();
-        if (tid < 16 && r0 + tid < n) g.ipiv[b * g.ipiv_stride + r0 + tid] = sPiv[r0 + tid];
+        if (tid < kLuBs && r0 + tid < n) g.ipiv[b * g.ipiv_stride + r0 + tid] = sPiv[r0 + tid];
 
-        if (tid < 16) {
-            // column tid of L_kk^{-1} (unit lower), forward substitution
-            double const* const d = sT[0][0];
-            double x[16];
+        if (tid < kLuBs) {
+            // column tid of L_kk^{-1} (32 x 32 unit lower), forward substitution
+            double x[kLuBs];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
+            for (int i = 0; i < kLuBs; ++i) {
                 double s = 0.0;
 #pragma unroll
-                for (int k = 0; k < i; ++k) s = __builtin_fma(-d[k * kTileStride + i], x[k], s);
+                for (int k = 0; k < i; ++k) s = __builtin_fma(-sP[i >> 4][k >> 4][(k & 15) * kTileStride + (i & 15)], x[k], s);
                 x[i] = (i == tid) ? 1.0 : (i < tid ? 0.0 : s);
             }
 #pragma unroll
-            for (int i = 0; i < 16; ++i) sWk[tid * kTileStride + i] = x[i];
+            for (int i = 0; i < kLuBs; ++i) sW[i >> 4][tid >> 4][(tid & 15) * kTileStride + (i & 15)] = x[i];
         }
         {
-            // the block's swaps on the columns right of the panel, 16 lanes
+            // the block's swaps on the columns right of the panel, 32 lanes
             // (one row each) per column: block row i becomes old row
-            // r0 + sSrc[i] (A_kj, to LDS) and a displaced row p_i >= r0 + 16
+            // r0 + sSrc[i] (A_kj, to LDS) and a displaced row p_i >= r0 + 32
             // becomes old row r0 + sSrc[p_i - r0] (back to A).  In each
             // column every load precedes every store.
-            int const i = lane & 15, grp = tid >> 4;
+            int const i = lane & 31, grp = tid >> 5;
             int const pi = sPiv[r0 + i];
-            bool const disp = pi >= r0 + 16;
+            bool const disp = pi >= r0 + kLuBs;
             int const srcb = r0 + sSrc[i], srcd = disp ? r0 + sSrc[pi - r0] : 0;
-            int const ncol = npad - r0 - 16;
-            constexpr int kU = 4;
-            for (int j0 = grp; j0 < ncol; j0 += 16 * kU) {
+            int const ncol = npad - r0 - kLuBs;
+            constexpr int kU = 4, kGroups = kLuThreads / kLuBs;
+            for (int j0 = grp; j0 < ncol; j0 += kGroups * kU) {
                 double xb[kU], xd[kU];
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
-                    int const j = r0 + 16 + j0 + 16 * u;
-                    bool const in = j0 + 16 * u < ncol;
+                    int const j = r0 + kLuBs + j0 + kGroups * u;
+                    bool const in = j0 + kGroups * u < ncol;
                     xb[u] = in ? fget(srcb, j) : 0.0;
                     xd[u] = in && disp ? fget(srcd, j) : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
-                    int const jr = j0 + 16 * u;  // column relative to the trailing block
+                    int const jr = j0 + kGroups * u;  // column relative to the trailing block
                     if (jr >= ncol) continue;
-                    sT[1][jr >> 4][i * kTileStride + (jr & 15)] = xb[u];
-                    if (disp) fset(pi, r0 + 16 + jr, xd[u]);
+                    sR[jr >> 4][i >> 4][(i & 15) * kTileStride + (jr & 15)] = xb[u];
+                    if (disp) fset(pi, r0 + kLuBs + jr, xd[u]);
                 }
             }
         }
         __syncthreads();
 
-        // ---- U_kj^T = A_kj^T L_kk^{-T}: to A (rows r0..) and to LDS as an A operand
+        // ---- U_kj^T = A_kj^T L_kk^{-T}: to A (rows r0..) and to LDS as an
+        // A operand.  U^T's first 16 columns take the block row's first row
+        // tile only (L_kk^{-1} is lower triangular).
         for (int jt = w; jt < mt; jt += kLuWaves) {
-            d4 acc = {0.0, 0.0, 0.0, 0.0};
+            d4 u0 = {0.0, 0.0, 0.0, 0.0}, u1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc = M::mma(op_read(sT[1][jt], q, lane), op_read(sWk, q, lane), acc);
+            for (int q = 0; q < 4; ++q) {
+                double const a0 = op_read(sR[jt][0], q, lane);
+                u0 = M::mma(a0, op_read(sW[0][0], q, lane), u0);
+                u1 = M::mma(a0, op_read(sW[1][0], q, lane), u1);
+            }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) fset(r0 + (lane & 15), 16 * (kb + 1 + jt) + M::row(lane, r), acc[r]);
-            acc_to_lds(sT[1][jt], acc, lane);
+            for (int q = 0; q < 4; ++q) u1 = M::mma(op_read(sR[jt][1], q, lane), op_read(sW[1][1], q, lane), u1);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                fset(r0 + (lane & 15), 16 * (T0 + jt) + M::row(lane, r), u0[r]);
+                fset(r0 + 16 + (lane & 15), 16 * (T0 + jt) + M::row(lane, r), u1[r]);
+            }
+            acc_to_lds(sR[jt][0], u0, lane);
+            acc_to_lds(sR[jt][1], u1, lane);
         }
         __syncthreads();
 
-        // ---- trailing A_ij^T -= U_kj^T L_ik^T, i, j > kb; wavefront w takes
-        // tiles w, w + 4, ..., kLuGroup at a time, the next group's loads
-        // issued before the current group's MFMAs
+        // ---- trailing A_ij^T -= U_kj^T L_ik^T, i, j >= T0 (16-tiles), K = 32;
+        // wavefront w takes tiles w, w + 4, ..., kLuGroup at a time, the next
+        // group's loads issued before the current group's MFMAs
         int const cnt = mt * mt;
         auto tile_of = [&](int t, int& ti, int& tj) {
-            ti = t < cnt ? kb + 1 + t % mt : -1;
-            tj = kb + 1 + (t < cnt ? t / mt : 0);
+            ti = t < cnt ? T0 + t % mt : -1;
+            tj = T0 + (t < cnt ? t / mt : 0);
         };
         auto load_group = [&](int t0, d4 (&acc)[kLuGroup]) {
 #pragma unroll
@@ -268,10 +297,13 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs g) {
                 int ti, tj;
                 tile_of(t0 + kLuWaves * u, ti, tj);
                 if (ti < 0) continue;
-                double const* const ut = sT[1][tj - kb - 1];
-                double const* const lt = sT[0][ti - kb];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) cur[u] = M::mma(-op_read(ut, q, lane), op_read(lt, q, lane), cur[u]);
+                for (int s2 = 0; s2 < 2; ++s2) {
+                    double const* const ut = sR[tj - T0][s2];
+                    double const* const lt = sP[ti - 2 * kb][s2];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) cur[u] = M::mma(-op_read(ut, q, lane), op_read(lt, q, lane), cur[u]);
+                }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) fset(16 * ti + (lane & 15), 16 * tj + M::row(lane, r), cur[u][r]);
             }
@@ -288,43 +320,46 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs g) {
     // Blocks are handled kChunk at a time: their gathers, one barrier, their
     // stores (different blocks touch different columns).
     int sig = tid;
-    constexpr int kChunk = 4;
-    for (int kb0 = NT - 2; kb0 >= 0; kb0 -= kChunk) {
-        double t[kChunk][16];
+    constexpr int kChunk = 2;
+    for (int kb0 = NB - 2; kb0 >= 0; kb0 -= kChunk) {
+        double t[kChunk][kLuBs];
         bool mv[kChunk];
 #pragma unroll
         for (int u = 0; u < kChunk; ++u) {
             int const kb = kb0 - u;
             if (kb >= 0) {
-                int const s0 = 16 * (kb + 1);
+                int const s0 = kLuBs * (kb + 1);
 #pragma unroll
-                for (int s = 15; s >= 0; --s) {
+                for (int s = kLuBs - 1; s >= 0; --s) {
                     int const rr = s0 + s, p = sPiv[rr];
                     sig = sig == rr ? p : (sig == p ? rr : sig);
                 }
             }
             mv[u] = kb >= 0 && tid < npad && sig != tid;
 #pragma unroll
-            for (int c = 0; c < 16; ++c) t[u][c] = mv[u] ? fget(sig, 16 * kb + c) : 0.0;
+            for (int c = 0; c < kLuBs; ++c) t[u][c] = mv[u] ? fget(sig, kLuBs * kb + c) : 0.0;
         }
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < kChunk; ++u)
             if (mv[u])
 #pragma unroll
-                for (int c = 0; c < 16; ++c) fset(tid, 16 * (kb0 - u) + c, t[u][c]);
+                for (int c = 0; c < kLuBs; ++c) fset(tid, kLuBs * (kb0 - u) + c, t[u][c]);
     }
 #pragma unroll
-    for (int s = 15; s >= 0; --s) {  // block 0's swaps: sig = the source row of row tid for B
+    for (int s = kLuBs - 1; s >= 0; --s) {  // block 0's swaps: sig = the source row of row tid for B
         int const p = sPiv[s];
         sig = sig == s ? p : (sig == p ? s : sig);
     }
+    __syncthreads();  // the solve's LDS overlays the factor's
 
     int const bad = sFlag;
     if (tid == 0 && g.info) g.info[b] = bad;
     if (bad || nrhs == 0) return;
 
     // ---- solve
+    auto& sT = sm.s.sT;
+    auto& sX = sm.s.sX;
     Ref<double> bb(g.B, b);
     if (tid < npad)
 #pragma unroll
