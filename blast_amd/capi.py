@@ -64,6 +64,7 @@ def lib():
             getattr(L, f"blast_hip_{p}potrf_batched").argtypes = [vp, P, P, i64, i32p]
             getattr(L, f"blast_hip_{p}getrf_batched").argtypes = [vp, P, vp, i64, i64, i32p]
         L.blast_hip_dposv_batched.argtypes = [vp, P, P, P, P, i64, i32p]
+        L.blast_hip_sposv_batched.argtypes = [vp, P, P, P, P, i64, i32p]
         L.blast_hip_dsposv_batched.argtypes = [vp, c_int, c_int, ctypes.c_double, P, P, P, i64, i32p, i32p]
         for p, real in (("d", ctypes.c_double), ("s", ctypes.c_float)):
             getattr(L, f"blast_hip_{p}gemm_backward_batched").argtypes = [vp, real, P, P, real, P, P, P, P, i64]
@@ -292,9 +293,9 @@ def getrf(A, ipiv, ipiv_stride, batch, info=None, stream=None):
 
 
 def posv(A, L, B, X, batch, info=None, stream=None):
-    name = "blast_hip_dposv_batched"
-    return check(name, lib().blast_hip_dposv_batched(_stream(stream), _ref(A), _ref(L), _ref(B), _ref(X), batch,
-                                                     _info_ptr(info)))
+    name = f"blast_hip_{_prefix(L.layout.dtype)}posv_batched"
+    return check(name, getattr(lib(), name)(_stream(stream), _ref(A), _ref(L), _ref(B), _ref(X), batch,
+                                            _info_ptr(info)))
 
 
 def sposv(precision, A, B, X, batch, iter_, info=None, itmax=30, tol=0.0, stream=None):

@@ -471,6 +471,10 @@ int blast_hip_dposv_batched(void* s, const blast_matrix_t* A, const blast_matrix
                             const blast_matrix_t* X, int64_t batch, int32_t* info) {
     return posv_entry<double>(s, A, L, B, X, batch, info);
 }
+int blast_hip_sposv_batched(void* s, const blast_matrix_t* A, const blast_matrix_t* L, const blast_matrix_t* B,
+                            const blast_matrix_t* X, int64_t batch, int32_t* info) {
+    return posv_entry<float>(s, A, L, B, X, batch, info);
+}
 
 int blast_hip_dsposv_batched(void* s, int precision, int itmax, double tol, const blast_matrix_t* A,
                              const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch, int32_t* info,

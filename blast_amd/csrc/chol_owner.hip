@@ -1,4 +1,5 @@
-// Fused batched Cholesky factor + solve, FP64, n <= 256, nrhs <= 8.
+// Fused batched Cholesky factor + solve, n <= 256, nrhs <= 8: FP64, and FP32
+// storage with FP64 arithmetic (loads widened, stores rounded).
 //
 // One workgroup of NW wavefronts per instance; the NT(NT+1)/2 lower 16x16
 // tiles are distributed round-robin over the wavefronts and stay in their
@@ -38,6 +39,38 @@ __device__ __forceinline__ int opaque(int v) {
     return v;
 }
 
+// Tile loads and stores in the accumulator layout for storage type T: FP64
+// storage uses the panel fast paths of tiles.hpp, FP32 storage (PS = 8
+// panels, dense, transposed) the generic element map; the arithmetic is FP64
+// either way (FP32 values are widened on load and rounded on store).
+template <bool PANEL, typename T>
+__device__ __forceinline__ d4 owner_load(Ref<T> const& a, int n, int ti, int tj, int lane) {
+    if constexpr (std::is_same_v<T, double>) {
+        return load_tile<PANEL, double>(a, n, ti, tj, lane);
+    } else {
+        d4 v;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int const row = 16 * ti + Mfma<double>::row(lane, r), col = 16 * tj + acc_col(lane);
+            v[r] = (row < n && col < n) ? (row >= col ? double(a(row, col)) : 0.0) : (row == col ? 1.0 : 0.0);
+        }
+        return v;
+    }
+}
+template <bool PANEL, typename T>
+__device__ __forceinline__ void owner_store(Ref<T> const& l, bool panel, int n, int ti, int tj, d4 v, int lane) {
+    if constexpr (std::is_same_v<T, double>) {
+        store_tile<PANEL>(l, panel, n, ti, tj, v, lane);
+    } else {  // the same elements as store_tile: lower part, zeros above it in 4x4 diagonal blocks (panel)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int const row = 16 * ti + Mfma<double>::row(lane, r), col = 16 * tj + acc_col(lane);
+            if (row < n && col < n && (row >= col || (panel && (row >> 2) == (col >> 2))))
+                l(row, col) = row >= col ? T(v[r]) : T(0);
+        }
+    }
+}
+
 // Shared memory of one instance with up to NT block columns.
 template <int NT>
 struct OwnerLds {
@@ -51,10 +84,9 @@ struct OwnerLds {
 // One instance (index b of the batch operands) by the NW wavefronts of the
 // workgroup; `lds` is at least OwnerLds<NT> (a larger one is reused by the
 // uneven kernel, whose instances have different NT).
-template <int NT, int NW, bool PANEL, int NTL>
-__device__ __forceinline__ void chol_owner_body(Mat<double> const& A, Mat<double> const& L, Mat<double> const& B,
-                                                Mat<double> const& X, int n, int nrhs, int64_t b, int32_t* info,
-                                                OwnerLds<NTL>& lds) {
+template <int NT, int NW, bool PANEL, int NTL, typename T = double>
+__device__ __forceinline__ void chol_owner_body(Mat<T> const& A, Mat<T> const& L, Mat<T> const& B, Mat<T> const& X,
+                                                int n, int nrhs, int64_t b, int32_t* info, OwnerLds<NTL>& lds) {
     static_assert(NT <= NTL);
     using Mf = Mfma<double>;
     constexpr int NTILES = NT * (NT + 1) / 2;
@@ -67,7 +99,7 @@ __device__ __forceinline__ void chol_owner_body(Mat<double> const& A, Mat<double
 
     int const lane = threadIdx.x & 63;
     int const w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    Ref<double> a(A, b), l(L, b);
+    Ref<T> a(A, b), l(L, b);
     bool const lpanel = L.panel != 0;
 
     // tile t = i(i+1)/2 + j lives in wavefront t % NW, register slot t / NW
@@ -81,15 +113,15 @@ __device__ __forceinline__ void chol_owner_body(Mat<double> const& A, Mat<double
         while (i * (i + 1) / 2 > t) --i;
         ti[s] = t < NTILES ? i : -1;
         tj[s] = t - i * (i + 1) / 2;
-        acc[s] = ti[s] >= 0 ? load_tile<PANEL, double>(a, n, ti[s], tj[s], lane) : d4{0.0, 0.0, 0.0, 0.0};
+        acc[s] = ti[s] >= 0 ? owner_load<PANEL>(a, n, ti[s], tj[s], lane) : d4{0.0, 0.0, 0.0, 0.0};
     }
     if (threadIdx.x == 0) sBad = 0;
     int const nr = nrhs;
     if (nr > 0) {
-        Ref<double> bb(B, b);
+        Ref<T> bb(B, b);
         for (int e = threadIdx.x; e < NT * 16 * kOwnRhs; e += 64 * NW) {
             int const row = e / kOwnRhs, c = e % kOwnRhs;
-            sR[e] = (row < n && c < nr) ? bb(row, c) : 0.0;
+            sR[e] = (row < n && c < nr) ? double(bb(row, c)) : 0.0;
         }
     }
     __syncthreads();
@@ -127,7 +159,7 @@ __device__ __forceinline__ void chol_owner_body(Mat<double> const& A, Mat<double
                 acc_to_lds(sD, dt, lane);
                 int const bk = diag_factor_inl<double>(sD, sW, lane);
                 if (lane == 0 && bk && sBad == 0 && 16 * kb + bk <= n) sBad = 16 * kb + bk;
-                store_tile<PANEL>(l, lpanel, n, kb, kb, lds_to_acc<double>(sD, lane), lane);
+                owner_store<PANEL>(l, lpanel, n, kb, kb, lds_to_acc<double>(sD, lane), lane);
                 d4 const wv = lds_to_acc<double>(sW, lane);
 #pragma unroll
                 for (int s = 0; s < S; ++s)
@@ -150,7 +182,7 @@ __device__ __forceinline__ void chol_owner_body(Mat<double> const& A, Mat<double
 #pragma unroll
                 for (int q = 0; q < 4; ++q) z = Mf::mma(op_read(sP[i], q, lane), op_read(sW, q, lane), z);
                 acc[s] = z;
-                store_tile<PANEL>(l, lpanel, n, i, opaque(kb), z, lane);
+                owner_store<PANEL>(l, lpanel, n, i, opaque(kb), z, lane);
                 acc_to_lds(sP[i], z, lane);
                 if (nr > 0) {
                     d4 r = rhs_load(i);
@@ -201,10 +233,10 @@ __device__ __forceinline__ void chol_owner_body(Mat<double> const& A, Mat<double
             }
         }
         __syncthreads();
-        Ref<double> xx(X, b);
+        Ref<T> xx(X, b);
         for (int e = threadIdx.x; e < n * nr; e += 64 * NW) {
             int const row = e % n, c = e / n;
-            xx(row, c) = sR[row * kOwnRhs + c];
+            xx(row, c) = T(sR[row * kOwnRhs + c]);
         }
     } else {
         __syncthreads();
@@ -212,14 +244,13 @@ __device__ __forceinline__ void chol_owner_body(Mat<double> const& A, Mat<double
     if (info && threadIdx.x == 0) info[b] = sBad;
 }
 
-template <int NT, int NW, int WPE, bool PANEL>
-__global__ __launch_bounds__(64 * NW, WPE) void chol_owner_kernel(Mat<double> A, Mat<double> L, Mat<double> B,
-                                                             Mat<double> X, int n, int nrhs, int32_t* info,
-                                                             const int32_t* only_if) {
+template <int NT, int NW, int WPE, bool PANEL, typename T = double>
+__global__ __launch_bounds__(64 * NW, WPE) void chol_owner_kernel(Mat<T> A, Mat<T> L, Mat<T> B, Mat<T> X, int n,
+                                                             int nrhs, int32_t* info, const int32_t* only_if) {
     __shared__ OwnerLds<NT> lds;
     int64_t const b = blockIdx.x;
     if (only_if && only_if[b] >= 0) return;
-    chol_owner_body<NT, NW, PANEL>(A, L, B, X, n, nrhs, b, info, lds);
+    chol_owner_body<NT, NW, PANEL, NT, T>(A, L, B, X, n, nrhs, b, info, lds);
 }
 
 // Uneven batches: one persistent launch over instances of different n (one
@@ -324,6 +355,41 @@ int launch_chol_owner(hipStream_t s, Mat<double> A, Mat<double> L, Mat<double> B
         case 14: return launch_owner_nt<14, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
         case 15: return launch_owner_nt<15, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
         default: return launch_owner_nt<16, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+    }
+}
+
+// FP32 storage (any layout), FP64 arithmetic: square n <= 256, nrhs <= 8; L
+// may alias A, X may alias B.  Returns 1 when the shape is not covered.
+template <int NT, int NW, int WPE = 1>
+int launch_owner_nt_f32(hipStream_t s, Mat<float> A, Mat<float> L, Mat<float> B, Mat<float> X, int n, int nrhs,
+                        int64_t batch, int32_t* info) {
+    hipLaunchKernelGGL((chol_owner_kernel<NT, NW, WPE, false, float>), dim3(unsigned(batch)), dim3(64 * NW), 0, s, A, L,
+                       B, X, n, nrhs, info, nullptr);
+    return check_launch("chol_owner_kernel<float>");
+}
+int launch_chol_owner_f32(hipStream_t s, Mat<float> A, Mat<float> L, Mat<float> B, Mat<float> X, int64_t batch,
+                          int32_t* info) {
+    int const n = L.rows, nrhs = (B.data && X.data) ? X.cols : 0;
+    if (L.rows != L.cols || n > 256 || nrhs > kOwnRhs) return 1;
+    if (batch == 0) return 0;
+    switch ((n + 15) / 16) {
+        case 0:
+        case 1: return launch_owner_nt_f32<1, 1>(s, A, L, B, X, n, nrhs, batch, info);
+        case 2: return launch_owner_nt_f32<2, 1>(s, A, L, B, X, n, nrhs, batch, info);
+        case 3: return launch_owner_nt_f32<3, 2>(s, A, L, B, X, n, nrhs, batch, info);
+        case 4: return launch_owner_nt_f32<4, 2>(s, A, L, B, X, n, nrhs, batch, info);
+        case 5: return launch_owner_nt_f32<5, 4>(s, A, L, B, X, n, nrhs, batch, info);
+        case 6: return launch_owner_nt_f32<6, 4>(s, A, L, B, X, n, nrhs, batch, info);
+        case 7: return launch_owner_nt_f32<7, 4>(s, A, L, B, X, n, nrhs, batch, info);
+        case 8: return launch_owner_nt_f32<8, 4, 2>(s, A, L, B, X, n, nrhs, batch, info);
+        case 9: return launch_owner_nt_f32<9, 8>(s, A, L, B, X, n, nrhs, batch, info);
+        case 10: return launch_owner_nt_f32<10, 8>(s, A, L, B, X, n, nrhs, batch, info);
+        case 11: return launch_owner_nt_f32<11, 8>(s, A, L, B, X, n, nrhs, batch, info);
+        case 12: return launch_owner_nt_f32<12, 8>(s, A, L, B, X, n, nrhs, batch, info);
+        case 13: return launch_owner_nt_f32<13, 8>(s, A, L, B, X, n, nrhs, batch, info);
+        case 14: return launch_owner_nt_f32<14, 8>(s, A, L, B, X, n, nrhs, batch, info);
+        case 15: return launch_owner_nt_f32<15, 8>(s, A, L, B, X, n, nrhs, batch, info);
+        default: return launch_owner_nt_f32<16, 8>(s, A, L, B, X, n, nrhs, batch, info);
     }
 }
 

@@ -52,12 +52,14 @@ int dispatch_trsm(hipStream_t s, int side_right, int upper, int unit, T alpha, M
 
 template <typename T>
 int dispatch_potrf(hipStream_t s, Mat<T> A, Mat<T> L, int64_t batch, int32_t* info) {
-    if constexpr (std::is_same_v<T, double>) {
-        if (algorithm() == BLAST_HIP_ALGO_AUTO) {
-            Mat<double> none{};
-            int const r = launch_chol_fused(s, A, L, none, none, batch, info);
-            if (r != 1) return r;
-        }
+    if (algorithm() == BLAST_HIP_ALGO_AUTO) {
+        Mat<T> none{};
+        int r;
+        if constexpr (std::is_same_v<T, double>)
+            r = launch_chol_fused(s, A, L, none, none, batch, info);
+        else
+            r = launch_chol_owner_f32(s, A, L, none, none, batch, info);
+        if (r != 1) return r;
     }
     return launch_potrf_ref<T>(s, A, L, batch, info);
 }
@@ -76,11 +78,13 @@ int dispatch_getrf(hipStream_t s, Mat<T> A, int64_t* ipiv, int64_t ipiv_stride, 
 
 template <typename T>
 int dispatch_posv(hipStream_t s, Mat<T> A, Mat<T> L, Mat<T> B, Mat<T> X, int64_t batch, int32_t* info) {
-    if constexpr (std::is_same_v<T, double>) {
-        if (algorithm() == BLAST_HIP_ALGO_AUTO && X.cols > 0) {
-            int const r = launch_chol_fused(s, A, L, B, X, batch, info);
-            if (r != 1) return r;
-        }
+    if (algorithm() == BLAST_HIP_ALGO_AUTO && X.cols > 0) {
+        int r;
+        if constexpr (std::is_same_v<T, double>)
+            r = launch_chol_fused(s, A, L, B, X, batch, info);
+        else
+            r = launch_chol_owner_f32(s, A, L, B, X, batch, info);
+        if (r != 1) return r;
     }
     int r = launch_potrf_ref<T>(s, A, L, batch, info);
     if (r) return r;

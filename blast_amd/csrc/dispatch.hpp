@@ -57,6 +57,10 @@ int launch_chol_fused(hipStream_t, Mat<double> A, Mat<double> L, Mat<double> B, 
 // returns 1 when the shape is not covered.  only_if as launch_chol_team.
 int launch_chol_owner(hipStream_t, Mat<double> A, Mat<double> L, Mat<double> B, Mat<double> X, int64_t batch,
                       int32_t* info, const int32_t* only_if = nullptr);
+// chol_owner.hip: the same for FP32 storage with FP64 arithmetic (loads
+// widened, stores rounded); n <= 256, nrhs <= 8, 1 when not covered.
+int launch_chol_owner_f32(hipStream_t, Mat<float> A, Mat<float> L, Mat<float> B, Mat<float> X, int64_t batch,
+                          int32_t* info);
 // chol_team.hip: fused FP64 Cholesky (+ solve) for n <= 128, nrhs <= 4, one
 // workgroup per instance; returns 1 when the shape is not covered.  only_if:
 // when non-null, only instances b with only_if[b] < 0 are processed.

@@ -521,10 +521,14 @@ void getrf(hip::Batch<T> A, std::int64_t* ipiv, std::int64_t ipiv_stride, std::i
         hip::check(blast_hip_sgetrf_batched(s, &A.desc, ipiv, ipiv_stride, A.count, info));
 }
 
-// potrf + trsm<Lower> + trsm<Upper>(trans(L)), fused (LAPACK dposv contract).
+// potrf + trsm<Lower> + trsm<Upper>(trans(L)), fused (LAPACK dposv / sposv contract).
 inline void posv(hip::Batch<double const> A, hip::Batch<double> L, hip::Batch<double const> B, hip::Batch<double> X,
                  std::int32_t* info = nullptr, hipStream_t s = nullptr) {
     hip::check(blast_hip_dposv_batched(s, &A.desc, &L.desc, &B.desc, &X.desc, X.count, info));
+}
+inline void posv(hip::Batch<float const> A, hip::Batch<float> L, hip::Batch<float const> B, hip::Batch<float> X,
+                 std::int32_t* info = nullptr, hipStream_t s = nullptr) {
+    hip::check(blast_hip_sposv_batched(s, &A.desc, &L.desc, &B.desc, &X.desc, X.count, info));
 }
 
 // Mixed precision (FP32 or BF16 factor) + FP64 refinement (LAPACK dsposv contract).

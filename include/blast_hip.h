@@ -139,6 +139,12 @@ int blast_hip_sgetrf_batched(void* stream, const blast_matrix_t* A, int64_t* ipi
  * lower triangle only. */
 int blast_hip_dposv_batched(void* stream, const blast_matrix_t* A, const blast_matrix_t* L,
                             const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch, int32_t* info);
+/* The same on FP32 storage (BLAST's float instantiation of the sequence).
+ * ALGO_AUTO, square n <= 256, nrhs <= 8: the arithmetic is FP64 inside
+ * (loads widened, L and X rounded to FP32 on store); other shapes and
+ * ALGO_REFERENCE: BLAST's FP32 operation order. */
+int blast_hip_sposv_batched(void* stream, const blast_matrix_t* A, const blast_matrix_t* L,
+                            const blast_matrix_t* B, const blast_matrix_t* X, int64_t batch, int32_t* info);
 
 /* Mixed-precision Cholesky solve with FP64 iterative refinement (the LAPACK
  * dsposv contract).  Not in BLAST; it solves the same systems as

@@ -271,3 +271,44 @@ def test_trsm_tiles_vs_oracle(dtype, tol, side_right, upper, unit, m, n, trans_a
     B = dev(lb, bh)
     capi.trsm(side_right, upper, unit, 1.5, dev(la, ah), B, B, batch)
     np.testing.assert_allclose(B.logical(batch), x_ref, rtol=0, atol=tol * max(1.0, np.abs(x_ref).max()))
+
+
+@pytest.mark.parametrize("n", [1, 7, 16, 33, 100, 128, 192, 256])
+@pytest.mark.parametrize("panel", [True, False])
+def test_float_posv_and_potrf_auto(n, panel):
+    """FP32 storage on the AUTO path (FP64 arithmetic inside, results rounded
+    to FP32): backward errors at FP32 rounding level, X against the FP64
+    solution, L only in its lower triangle (zeros above it inside the 4x4
+    diagonal blocks of panel storage, as the reference-order kernel writes)."""
+    import torch
+
+    from tests.numerics import EPS32
+
+    rng = np.random.default_rng(n + 1000 * panel)
+    batch, nrhs = 5, min(8, max(1, n // 16))
+    a = spd(rng, batch, n).astype(np.float32)
+    b = rng.uniform(-1, 1, (batch, n, nrhs)).astype(np.float32)
+    la, lb = Layout.make(n, n, np.float32, panel=panel), Layout.make(n, nrhs, np.float32, panel=not panel)
+    ah, bh = la.pack(a), lb.pack(b)
+    L, X = capi.Operand.empty(la, batch, fill=3.0), capi.Operand.empty(lb, batch)
+    info = torch.full((batch,), -1, dtype=torch.int32, device="cuda")
+    capi.posv(dev(la, ah), L, dev(lb, bh), X, batch, info)
+    assert np.all(info.cpu().numpy() == 0)
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    lg, x = L.logical(batch).astype(np.float64), X.logical(batch).astype(np.float64)
+    assert np.all(potrf_backward_error(a64, lg) < 4 * EPS32)
+    assert np.all(solve_backward_error(a64, x, b64) < 4 * EPS32)
+    x_ref = np.linalg.solve(a64, b64)
+    assert np.all(np.abs(x - x_ref).max(axis=(1, 2)) <= 8 * EPS32 * np.linalg.cond(a64) * np.abs(x_ref).max(axis=(1, 2)))
+    low = np.tril(np.ones((n, n), bool))
+    blk = np.zeros((n, n), bool)
+    for i in range(n):
+        for j in range(i + 1, n):
+            blk[i, j] = panel and i // 4 == j // 4
+    assert np.all(lg[:, blk] == 0.0)
+    assert np.all(lg[:, ~low & ~blk] == 3.0)  # untouched
+    # potrf alone, in place
+    A = dev(la, ah)
+    capi.potrf(A, A, batch, info)
+    assert np.all(info.cpu().numpy() == 0)
+    np.testing.assert_array_equal(A.logical(batch)[:, low], L.logical(batch)[:, low])
