@@ -66,11 +66,13 @@ int dispatch_potrf(hipStream_t s, Mat<T> A, Mat<T> L, int64_t batch, int32_t* in
 
 template <typename T>
 int dispatch_getrf(hipStream_t s, Mat<T> A, int64_t* ipiv, int64_t ipiv_stride, int64_t batch, int32_t* info) {
-    if constexpr (std::is_same_v<T, double>) {
-        // column-major storage (partial pivoting); the row-major form has no pivoting
-        if (algorithm() == BLAST_HIP_ALGO_AUTO && !A.trans && A.rows == A.cols && wg_covers(A.rows, 0)) {
+    // column-major storage (partial pivoting); the row-major form has no pivoting
+    if (algorithm() == BLAST_HIP_ALGO_AUTO && !A.trans && A.rows == A.cols && wg_covers(A.rows, 0)) {
+        if constexpr (std::is_same_v<T, double>) {
             Mat<double> none{};
             return launch_wg_gesv(s, A, ipiv, ipiv_stride, none, none, 0, batch, info);
+        } else {
+            return launch_wg_getrf_f32(s, A, ipiv, ipiv_stride, batch, info);
         }
     }
     return launch_getrf_ref<T>(s, A, ipiv, ipiv_stride, batch, info);

@@ -87,6 +87,8 @@ bool wg_covers(int n, int nrhs);
 int64_t wg_work_elems(int n);  // floats per instance of the mixed-precision factor
 int launch_wg_gesv(hipStream_t, Mat<double> A, int64_t* ipiv, int64_t ipiv_stride, Mat<double> B, Mat<double> X,
                    int nrhs, int64_t batch, int32_t* info, const int32_t* only_if = nullptr);
+int launch_wg_getrf_f32(hipStream_t, Mat<float> A, int64_t* ipiv, int64_t ipiv_stride, int64_t batch,
+                        int32_t* info);  // FP32 storage, FP64 arithmetic
 int launch_wg_mixed(hipStream_t, int lu, int bf16, int itmax, double tol, Mat<double> A, int64_t* ipiv,
                     int64_t ipiv_stride, Mat<double> B, Mat<double> X, int nrhs, float* work, int64_t batch,
                     int32_t* info, int32_t* iter);

@@ -91,8 +91,11 @@ __device__ __forceinline__ void wave_argmax(double& v, int& i) {
     argmax_swap<false>(v, i);
 }
 
+// T: the storage type (the arithmetic is FP64 for FP32 storage too: values
+// are widened on load and rounded on store).
+template <typename T>
 struct LuArgs {
-    Mat<double> A, B, X;
+    Mat<T> A, B, X;
     int n, nrhs;
     int32_t* info;
     const int32_t* only_if;
@@ -117,7 +120,8 @@ struct LuSolveLds {
     double sX[kLuMaxNT * 16 * kLuRhs];     // right-hand sides, [row][rhs]
 };
 
-__global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs g) {
+template <typename T>
+__global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs<T> g) {
     using M = Mfma<double>;
     __shared__ union {
         LuFactorLds f;
@@ -138,11 +142,11 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs g) {
     int const tid = threadIdx.x, lane = tid & 63;
     int const w = __builtin_amdgcn_readfirstlane(tid >> 6);
     int const n = g.n, nrhs = g.nrhs, NB = (n + kLuBs - 1) / kLuBs, npad = kLuBs * NB, NT = 2 * NB;
-    Ref<double> f(g.A, b);
+    Ref<T> f(g.A, b);
     // A outside n x n reads as the identity (the factorisation of blkdiag(A, I))
-    auto fget = [&](int i, int j) -> double { return (i < n && j < n) ? f(i, j) : (i == j ? 1.0 : 0.0); };
+    auto fget = [&](int i, int j) -> double { return (i < n && j < n) ? double(f(i, j)) : (i == j ? 1.0 : 0.0); };
     auto fset = [&](int i, int j, double v) {
-        if (i < n && j < n) f(i, j) = v;
+        if (i < n && j < n) f(i, j) = T(v);
     };
     if (tid == 0) sFlag = 0;
 
@@ -360,10 +364,10 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs g) {
     // ---- solve
     auto& sT = sm.s.sT;
     auto& sX = sm.s.sX;
-    Ref<double> bb(g.B, b);
+    Ref<T> bb(g.B, b);
     if (tid < npad)
 #pragma unroll
-        for (int c = 0; c < kLuRhs; ++c) sX[tid * kLuRhs + c] = (tid < n && c < nrhs) ? bb(sig, c) : 0.0;
+        for (int c = 0; c < kLuRhs; ++c) sX[tid * kLuRhs + c] = (tid < n && c < nrhs) ? double(bb(sig, c)) : 0.0;
     for (int e = tid; e < NT * 256; e += kLuThreads) {  // the diagonal blocks
         int const k = e >> 8, c = (e >> 4) & 15, m = e & 15;
         sT[2][k][c * kTileStride + m] = fget(16 * k + m, 16 * k + c);
@@ -473,10 +477,10 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs g) {
         }
         __syncthreads();
     }
-    Ref<double> x(g.X, b);
+    Ref<T> x(g.X, b);
     for (int e = tid; e < n * nrhs; e += kLuThreads) {
         int const r = e % n, c = e / n;
-        x(r, c) = sX[r * kLuRhs + c];
+        x(r, c) = T(sX[r * kLuRhs + c]);
     }
 }
 
@@ -487,13 +491,26 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs g) {
 int launch_wg_gesv(hipStream_t s, Mat<double> A, int64_t* ipiv, int64_t ipiv_stride, Mat<double> B, Mat<double> X,
                    int nrhs, int64_t batch, int32_t* info, const int32_t* only_if) {
     if (batch == 0) return 0;
-    LuArgs g;
+    LuArgs<double> g;
     g.A = A, g.B = B, g.X = X;
     g.n = A.rows, g.nrhs = nrhs;
     g.info = info, g.only_if = only_if;
     g.ipiv = ipiv, g.ipiv_stride = ipiv_stride;
-    hipLaunchKernelGGL(lu_wg_kernel, dim3(unsigned(batch)), dim3(kLuThreads), 0, s, g);
+    hipLaunchKernelGGL(lu_wg_kernel<double>, dim3(unsigned(batch)), dim3(kLuThreads), 0, s, g);
     return check_launch("lu_wg_kernel");
+}
+
+// The LU alone on FP32 storage (FP64 arithmetic), pivots to ipiv.
+int launch_wg_getrf_f32(hipStream_t s, Mat<float> A, int64_t* ipiv, int64_t ipiv_stride, int64_t batch,
+                        int32_t* info) {
+    if (batch == 0) return 0;
+    LuArgs<float> g;
+    g.A = A, g.B = Mat<float>{}, g.X = Mat<float>{};
+    g.n = A.rows, g.nrhs = 0;
+    g.info = info, g.only_if = nullptr;
+    g.ipiv = ipiv, g.ipiv_stride = ipiv_stride;
+    hipLaunchKernelGGL(lu_wg_kernel<float>, dim3(unsigned(batch)), dim3(kLuThreads), 0, s, g);
+    return check_launch("lu_wg_kernel<float>");
 }
 
 }  // namespace blast_hip

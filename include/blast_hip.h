@@ -120,6 +120,8 @@ int blast_hip_strsv_batched(void* stream, int upper, const blast_matrix_t* A, co
  * BLAST and are reported in info. */
 int blast_hip_dpotrf_batched(void* stream, const blast_matrix_t* A, const blast_matrix_t* L, int64_t batch,
                              int32_t* info);
+/* FP32 storage: on ALGO_AUTO, square n <= 256 is factored with FP64
+ * arithmetic inside (L rounded to FP32 on store). */
 int blast_hip_spotrf_batched(void* stream, const blast_matrix_t* A, const blast_matrix_t* L, int64_t batch,
                              int32_t* info);
 
@@ -130,6 +132,9 @@ int blast_hip_spotrf_batched(void* stream, const blast_matrix_t* A, const blast_
  * (include/blast/math/dense/Getrf.hpp:52-119 and :141-162). */
 int blast_hip_dgetrf_batched(void* stream, const blast_matrix_t* A, int64_t* ipiv, int64_t ipiv_stride,
                              int64_t batch, int32_t* info);
+/* FP32 storage: on ALGO_AUTO a square column-major A with n <= 256 is
+ * factored with FP64 arithmetic inside (factors rounded to FP32); other
+ * shapes and ALGO_REFERENCE follow BLAST's FP32 operation order. */
 int blast_hip_sgetrf_batched(void* stream, const blast_matrix_t* A, int64_t* ipiv, int64_t ipiv_stride,
                              int64_t batch, int32_t* info);
 

@@ -249,3 +249,31 @@ def test_mixed_uncovered_shape_and_missing_work():
         capi.sposv_ws(0, A2, capi.Operand.empty(lb2, 1), capi.Operand.empty(lb2, 1), None, 1,
                       torch.zeros(1, dtype=torch.int32, device="cuda"))
     assert e.value.code == -8
+
+
+@pytest.mark.parametrize("n", [5, 48, 129, 256])
+@pytest.mark.parametrize("panel", [False, True])
+def test_float_getrf_auto(n, panel):
+    """FP32 storage on the AUTO path (FP64 arithmetic inside, factors rounded
+    to FP32): pivots equal the FP64 oracle's on the widened input
+    (well-conditioned, so no near-ties), backward error at FP32 rounding."""
+    import torch
+
+    from tests.numerics import EPS32
+
+    rng = np.random.default_rng(n + 7)
+    batch = 3
+    a = (rng.uniform(-1, 1, (batch, n, n)) + 0.5 * np.eye(n)).astype(np.float32)
+    la32, la64 = Layout.make(n, n, np.float32, panel=panel), Layout.make(n, n, np.float64, panel=panel)
+    ref = la64.pack(a.astype(np.float64))
+    piv_ref, inf_ref = oracle.getrf(la64, ref, batch)
+    A = dev(la32, la32.pack(a))
+    ipiv = torch.full((batch * n,), -1, dtype=torch.int64, device="cuda")
+    info = i32(batch, -1)
+    capi.getrf(A, ipiv, n, batch, info)
+    piv = ipiv.cpu().numpy().reshape(batch, n)
+    np.testing.assert_array_equal(piv, piv_ref)
+    np.testing.assert_array_equal(info.cpu().numpy(), inf_ref)
+    lu = A.logical(batch).astype(np.float64)
+    for k in range(batch):
+        assert lu_backward_error(a[k].astype(np.float64), lu[k], piv[k]) < 8 * EPS32
