@@ -70,9 +70,9 @@ int dispatch_getrf(hipStream_t s, Mat<T> A, int64_t* ipiv, int64_t ipiv_stride, 
     if (algorithm() == BLAST_HIP_ALGO_AUTO && !A.trans && A.rows == A.cols && wg_covers(A.rows, 0)) {
         if constexpr (std::is_same_v<T, double>) {
             Mat<double> none{};
-            return launch_wg_gesv(s, A, ipiv, ipiv_stride, none, none, 0, batch, info);
+            if (lu_wg_fits(A, none, 0)) return launch_wg_gesv(s, A, ipiv, ipiv_stride, none, none, 0, batch, info);
         } else {
-            return launch_wg_getrf_f32(s, A, ipiv, ipiv_stride, batch, info);
+            if (lu_wg_fits_f32(A)) return launch_wg_getrf_f32(s, A, ipiv, ipiv_stride, batch, info);
         }
     }
     return launch_getrf_ref<T>(s, A, ipiv, ipiv_stride, batch, info);
@@ -126,7 +126,7 @@ int dispatch_dsposv_ws(hipStream_t s, int bf16, int itmax, double tol, Mat<doubl
 // LU with partial pivoting in place of A and the solve (LAPACK dgesv).
 int dispatch_dgesv(hipStream_t s, Mat<double> A, int64_t* ipiv, int64_t ipiv_stride, Mat<double> B, Mat<double> X,
                    int64_t batch, int32_t* info) {
-    if (algorithm() == BLAST_HIP_ALGO_AUTO && wg_covers(A.rows, X.cols))
+    if (algorithm() == BLAST_HIP_ALGO_AUTO && wg_covers(A.rows, X.cols) && lu_wg_fits(A, X, X.cols))
         return launch_wg_gesv(s, A, ipiv, ipiv_stride, B, X, X.cols, batch, info);
     int const r = launch_getrf_ref<double>(s, A, ipiv, ipiv_stride, batch, info);
     if (r) return r;
@@ -137,7 +137,7 @@ int dispatch_dgesv(hipStream_t s, Mat<double> A, int64_t* ipiv, int64_t ipiv_str
 // not converge are solved by the FP64 LU in place of A.
 int dispatch_dsgesv(hipStream_t s, int bf16, int itmax, double tol, Mat<double> A, int64_t* ipiv, int64_t ipiv_stride,
                     Mat<double> B, Mat<double> X, float* work, int64_t batch, int32_t* info, int32_t* iter) {
-    if (wg_covers(A.rows, X.cols) && work) {
+    if (wg_covers(A.rows, X.cols) && work && lu_wg_fits(A, X, X.cols)) {
         int const r =
             launch_wg_mixed(s, 1, bf16, itmax, tol, A, ipiv, ipiv_stride, B, X, X.cols, work, batch, info, iter);
         if (r) return r;

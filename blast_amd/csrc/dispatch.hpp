@@ -89,6 +89,8 @@ int launch_wg_gesv(hipStream_t, Mat<double> A, int64_t* ipiv, int64_t ipiv_strid
                    int nrhs, int64_t batch, int32_t* info, const int32_t* only_if = nullptr);
 int launch_wg_getrf_f32(hipStream_t, Mat<float> A, int64_t* ipiv, int64_t ipiv_stride, int64_t batch,
                         int32_t* info);  // FP32 storage, FP64 arithmetic
+bool lu_wg_fits(Mat<double> A, Mat<double> X, int nrhs);  // the LU kernel's 32-bit offsets cover A and X
+bool lu_wg_fits_f32(Mat<float> A);
 int launch_wg_mixed(hipStream_t, int lu, int bf16, int itmax, double tol, Mat<double> A, int64_t* ipiv,
                     int64_t ipiv_stride, Mat<double> B, Mat<double> X, int nrhs, float* work, int64_t batch,
                     int32_t* info, int32_t* iter);

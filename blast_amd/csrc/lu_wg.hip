@@ -55,7 +55,7 @@ constexpr int kLuWaves = kLuThreads / 64;
 constexpr int kLuMaxNT = 16;  // n <= 256: one panel row per thread
 constexpr int kLuBs = 32;     // block column width (two 16x16 tiles)
 constexpr int kLuRhs = 8;     // right-hand-side columns per instance
-constexpr int kLuGroup = 8;   // trailing tiles per wavefront in flight
+constexpr int kLuGroup = 5;   // trailing tiles per wavefront in flight
 
 // One step of the wavefront argmax: combine with the lane that DPP control
 // CTRL names (quad_perm, row_half_mirror, row_mirror: all within a row).
@@ -103,25 +103,64 @@ struct LuArgs {
     int64_t ipiv_stride;
 };
 
-// LDS of one instance: the factor phase's tiles and the solve phase's
-// overlay the same storage.  Tiles are 16x16 with element (row m, col c) at
-// c*17 + m unless noted.
+// LDS of one instance (80.4 KB: two instances per CU); the factor phase's
+// tiles and the solve phase's overlay the same storage.  Tiles are 16x16
+// with element (row m, col c) at c*17 + m.
 struct LuFactorLds {
-    double sP[kLuMaxNT][2][kTileElems];      // panel: tile (row tile t, column tile s) of rows r0 + 16 t..
-    double sR[kLuMaxNT - 2][2][kTileElems];  // block row, column tile t, row tile s: A_kj (element (k, m)
-                                             // at k*17 + m), then U_kj^T (accumulator layout)
-    double sW[2][2][kTileElems];             // L_kk^{-1} (unit lower, 32 x 32)
-    double sRow[2][kLuWaves][kLuBs];         // each wavefront's candidate pivot row
+    double sP[kLuMaxNT][2][kTileElems];  // panel: tile (row tile t, column tile s) of rows r0 + 16 t..
+    double sW[3][kTileElems];            // L_kk^{-1} (unit lower, 32 x 32): W00, W10, W11
+    double sRow[2][kLuWaves][kLuBs];     // each wavefront's candidate pivot row
     double sCandV[2][kLuWaves];
     int sCandI[2][kLuWaves];
+    int sNonFinite[kLuWaves];
 };
 struct LuSolveLds {
-    double sT[3][kLuMaxNT][kTileElems];    // unit-lower inverses, upper inverses, diagonal blocks
-    double sX[kLuMaxNT * 16 * kLuRhs];     // right-hand sides, [row][rhs]
+    double sInv[kLuMaxNT][kTileElems];  // U_kk^{-1}
+    double sX[kLuMaxNT * 16 * kLuRhs];  // right-hand sides, [row][rhs]
 };
 
+// Element offsets of one instance's storage in 32 bits (the launcher checks
+// that every offset of the matrix fits), as row part + column part:
+// off(i, j) = rows(i) + cols(j), the panel or dense map of Mat::off.  Each
+// part is (x >> s) * a + (x & msk) * c with per-matrix constants: the panel
+// index (x / PS) spacing + x % PS, the dense row x, or the column x * PS /
+// x * spacing, with no branch on the storage kind.
+struct AxisMap {
+    int s, a, msk, c;
+    __device__ __forceinline__ int operator()(int x) const { return (x >> s) * a + (x & msk) * c; }
+};
 template <typename T>
-__global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs<T> g) {
+struct Sep {
+    T* p;
+    AxisMap rows, cols;
+    // a 32-bit byte offset from the instance base: the uniform base stays in
+    // SGPRs and every address costs one VGPR
+    __device__ __forceinline__ T& at(int off) const {
+        return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + unsigned(off) * unsigned(sizeof(T)));
+    }
+    static __device__ __forceinline__ Sep make(Mat<T> const& m, T* p) {
+        constexpr int PS = PanelSize<T>::value, LOG = PanelSize<T>::log2;
+        AxisMap const P = m.panel ? AxisMap{LOG, m.spacing, PS - 1, 1} : AxisMap{31, 0, 0x7fffffff, 1};
+        AxisMap const Q{31, 0, 0x7fffffff, m.panel ? PS : m.spacing};
+        return Sep{p, m.trans ? Q : P, m.trans ? P : Q};
+    }
+};
+
+// The value of a load, opaque to the compiler: a select on it cannot be turned
+// into a branch around the load (which would issue the loads one at a time).
+__device__ __forceinline__ double pin(double x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+// B operand of a 16x16 LDS tile M (element (m, c) at c*17 + m) for K-chunk q:
+// M[4q + l/16][l%16].
+__device__ __forceinline__ double op_read_t(const double* tile, int q, int lane) {
+    return tile[(lane & 15) * kTileStride + 4 * q + (lane >> 4)];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kLuThreads, 2) void lu_wg_kernel(LuArgs<T> g) {
     using M = Mfma<double>;
     __shared__ union {
         LuFactorLds f;
@@ -131,7 +170,6 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs<T> g) {
     __shared__ int sSrc[kLuMaxNT * 16];  // panel position -> its row before the block's swaps
     __shared__ int sFlag;
     auto& sP = sm.f.sP;
-    auto& sR = sm.f.sR;
     auto& sW = sm.f.sW;
     auto& sRow = sm.f.sRow;
     auto& sCandV = sm.f.sCandV;
@@ -141,13 +179,33 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs<T> g) {
     if (g.only_if && g.only_if[b] >= 0) return;
     int const tid = threadIdx.x, lane = tid & 63;
     int const w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int const m = lane & 15, gq = lane >> 4;
     int const n = g.n, nrhs = g.nrhs, NB = (n + kLuBs - 1) / kLuBs, npad = kLuBs * NB, NT = 2 * NB;
-    Ref<T> f(g.A, b);
-    // A outside n x n reads as the identity (the factorisation of blkdiag(A, I))
-    auto fget = [&](int i, int j) -> double { return (i < n && j < n) ? double(f(i, j)) : (i == j ? 1.0 : 0.0); };
-    auto fset = [&](int i, int j, double v) {
-        if (i < n && j < n) f(i, j) = T(v);
+    bool const rhs = nrhs > 0;
+    Sep<T> const sa = Sep<T>::make(g.A, g.A.inst(b));
+    // A outside n x n reads as the identity (the factorisation of blkdiag(A,
+    // I)); loads are clamped into the matrix and their value replaced
+    auto fget = [&](int i, int j) -> double {
+        double const v = double(sa.at(sa.rows(min(i, n - 1)) + sa.cols(min(j, n - 1))));
+        return (i < n && j < n) ? v : (i == j ? 1.0 : 0.0);
     };
+    auto fset = [&](int i, int j, double v) {
+        if (i < n && j < n) sa.at(sa.rows(i) + sa.cols(j)) = T(v);
+    };
+    // the right-hand sides ride along as column tile NT of [A | X]: the row
+    // swaps and the elimination turn X = B into y = L^{-1} P B
+    Sep<T> const sx = Sep<T>::make(g.X, rhs ? g.X.inst(b) : nullptr);
+    auto xget = [&](int i, int c) -> double {
+        double const v = double(sx.at(sx.rows(min(i, n - 1)) + sx.cols(min(c, nrhs - 1))));
+        return (i < n && c < nrhs) ? v : 0.0;
+    };
+    if (rhs && g.X.data != g.B.data) {  // X = B (X may alias B)
+        Ref<T> bb(g.B, b);
+        for (int e = tid; e < n * nrhs; e += kLuThreads) {
+            int const r = e % n, c = e / n;
+            sx.at(sx.rows(r) + sx.cols(c)) = bb(r, c);
+        }
+    }
     if (tid == 0) sFlag = 0;
 
     for (int kb = 0; kb < NB; ++kb) {
@@ -159,161 +217,237 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs<T> g) {
         bool const live = tid < P;
         int pos = tid;
         double v[kLuBs];
+        // full: every column of the block lies inside the matrix (uniform),
+        // so that per-column conditions are needed only in the last block
+        bool const full = r0 + kLuBs <= n;
+        {
+            // (the identity fill of the padding is formed here, not hoisted
+            // out of the block loop as 32 loop-invariant registers)
+            int const me = opaque_v(tid);
+            int const rowoff = sa.rows(min(r0 + me, n - 1));
+            bool const rowok = r0 + me < n;
 #pragma unroll
-        for (int c = 0; c < kLuBs; ++c) v[c] = live ? fget(r0 + tid, r0 + c) : 0.0;
+            for (int c = 0; c < kLuBs; ++c) v[c] = double(sa.at(rowoff + sa.cols(min(r0 + c, n - 1))));
+            if (full) {
 #pragma unroll
-        for (int c = 0; c < kLuBs; ++c) {
-            int const buf = c & 1;
-            bool const cand = live && pos >= c;
-            double kv = cand ? iamax_key(v[c], pos - c) : -1.0;
-            int ki = cand ? pos : 0x7fffffff;
-            wave_argmax(kv, ki);
-            if (pos == ki)
+                for (int c = 0; c < kLuBs; ++c) v[c] = rowok ? pin(v[c]) : (me == c && live ? 1.0 : 0.0);
+            } else {
 #pragma unroll
-                for (int cc = c; cc < kLuBs; ++cc) sRow[buf][w][cc] = v[cc];
-            if (lane == 0) sCandV[buf][w] = kv, sCandI[buf][w] = ki;
-            __syncthreads();
-            double pv = sCandV[buf][0];
-            int pp = sCandI[buf][0], wp = 0;
-#pragma unroll
-            for (int q = 1; q < kLuWaves; ++q) {
-                int const before = pp;
-                argmax_combine(pv, pp, sCandV[buf][q], sCandI[buf][q]);
-                wp = pp != before ? q : wp;
-            }
-            // position c always holds a candidate, so pp is a real position
-            pos = pos == pp ? c : (pos == c ? pp : pos);
-            if (tid == 0) sPiv[r0 + c] = r0 + pp;
-            double const piv = sRow[buf][wp][c];
-            if (piv == 0.0) {
-                if (tid == 0 && sFlag == 0 && r0 + c < n) sFlag = r0 + c + 1;
-            } else if (live && pos > c) {
-                double const l = v[c] / piv;
-                v[c] = l;
-#pragma unroll
-                for (int cc = c + 1; cc < kLuBs; ++cc) v[cc] = __builtin_fma(-l, sRow[buf][wp][cc], v[cc]);
+                for (int c = 0; c < kLuBs; ++c)
+                    v[c] = (rowok && r0 + c < n) ? pin(v[c]) : (me == c && live ? 1.0 : 0.0);
             }
         }
+        // The update of a column runs on every row, finished ones with a zero
+        // multiplier (no per-lane branch, no selects); that leaves them exact
+        // unless the pivot row holds an infinity or a NaN, so a panel with a
+        // non-finite entry takes the masked update instead.
+        bool nf = false;
+#pragma unroll
+        for (int c = 0; c < kLuBs; ++c) nf |= !(__builtin_fabs(v[c]) <= __DBL_MAX__);
+        if (lane == 0) sm.f.sNonFinite[w] = __ballot(nf) != 0;
+        __syncthreads();
+        bool const masked = sm.f.sNonFinite[0] | sm.f.sNonFinite[1] | sm.f.sNonFinite[2] | sm.f.sNonFinite[3];
+        auto panel = [&](auto MASKED) __attribute__((always_inline)) {
+            constexpr bool kMasked = decltype(MASKED)::value;
+            static_for<0, kLuBs>([&](auto C) __attribute__((always_inline)) {
+                constexpr int c = decltype(C)::value, buf = c & 1;
+                bool const cand = live && pos >= c;
+                double kv = cand ? iamax_key(v[c], pos - c) : -1.0;
+                int ki = cand ? pos : 0x7fffffff;
+                wave_argmax(kv, ki);
+                if (pos == ki)
+#pragma unroll
+                    for (int cc = c; cc < kLuBs; ++cc) sRow[buf][w][cc] = v[cc];
+                if (lane == 0) sCandV[buf][w] = kv, sCandI[buf][w] = ki;
+                __syncthreads();
+                double pv = sCandV[buf][0];
+                int pp = sCandI[buf][0], wp = 0;
+#pragma unroll
+                for (int q = 1; q < kLuWaves; ++q) {
+                    int const before = pp;
+                    argmax_combine(pv, pp, sCandV[buf][q], sCandI[buf][q]);
+                    wp = pp != before ? q : wp;
+                }
+                // position c always holds a candidate, so pp is a real position
+                pos = pos == pp ? c : (pos == c ? pp : pos);
+                if (tid == 0) sPiv[r0 + c] = r0 + pp;
+                double const* const prow = sRow[buf][wp];
+                double const piv = prow[c];
+                if (piv == 0.0 && tid == 0 && sFlag == 0 && r0 + c < n) sFlag = r0 + c + 1;
+                bool const upd = live && pos > c && piv != 0.0;
+                double const l = v[c] / piv;
+                if constexpr (kMasked) {
+                    if (upd) {
+                        v[c] = l;
+#pragma unroll
+                        for (int cc = c + 1; cc < kLuBs; ++cc) v[cc] = __builtin_fma(-l, prow[cc], v[cc]);
+                    }
+                } else {
+                    double const lm = upd ? l : 0.0;
+                    v[c] = upd ? l : v[c];
+#pragma unroll
+                    for (int cc = c + 1; cc < kLuBs; ++cc) v[cc] = __builtin_fma(-lm, prow[cc], v[cc]);
+                }
+                // the next column's search starts after this update (no
+                // overlap: keeps the pivot row's 31 values and the next
+                // search's registers from being live together)
+                __builtin_amdgcn_sched_barrier(0);
+            });
+        };
+        if (masked)
+            panel(std::true_type{});
+        else
+            panel(std::false_type{});
         if (live) {
             sSrc[pos] = tid;
 #pragma unroll
-            for (int cc = 0; cc < kLuBs; ++cc) {
-                fset(r0 + pos, r0 + cc, v[cc]);
-                sP[pos >> 4][cc >> 4][(cc & 15) * kTileStride + (pos & 15)] = v[cc];
+            for (int cc = 0; cc < kLuBs; ++cc) sP[pos >> 4][cc >> 4][(cc & 15) * kTileStride + (pos & 15)] = v[cc];
+            if (r0 + pos < n) {
+                int const rowoff = sa.rows(r0 + pos);
+                if (full) {
+#pragma unroll
+                    for (int cc = 0; cc < kLuBs; ++cc) sa.at(rowoff + sa.cols(r0 + cc)) = T(v[cc]);
+                } else {
+#pragma unroll
+                    for (int cc = 0; cc < kLuBs; ++cc)
+                        if (r0 + cc < n) sa.at(rowoff + sa.cols(r0 + cc)) = T(v[cc]);
+                }
             }
         }
         __syncthreads();
         if (tid < kLuBs && r0 + tid < n) g.ipiv[b * g.ipiv_stride + r0 + tid] = sPiv[r0 + tid];
 
-        if (tid < kLuBs) {
-            // column tid of L_kk^{-1} (32 x 32 unit lower), forward substitution
-            double x[kLuBs];
+        // ---- W = L_kk^{-1} (32 x 32 unit lower): the diagonal 16x16 blocks'
+        // inverses on two wavefronts (tri_inv16_regs: lane (g, i) forms row
+        // i's columns 4u + g), then W10 = -W11 (L10 W00) on MFMA
+        if (w < 2) {
+            double const* const lt = sP[w][w];
+            double* const wt = sW[2 * w];
+            int const i = lane & 15;
+            double wv[4];
+            tri_inv16_regs(wv, lane, 1.0, [&](int j) { return j < i ? -lt[j * kTileStride + i] : 0.0; }, NoHook{});
 #pragma unroll
-            for (int i = 0; i < kLuBs; ++i) {
-                double s = 0.0;
-#pragma unroll
-                for (int k = 0; k < i; ++k) s = __builtin_fma(-sP[i >> 4][k >> 4][(k & 15) * kTileStride + (i & 15)], x[k], s);
-                x[i] = (i == tid) ? 1.0 : (i < tid ? 0.0 : s);
-            }
-#pragma unroll
-            for (int i = 0; i < kLuBs; ++i) sW[i >> 4][tid >> 4][(tid & 15) * kTileStride + (i & 15)] = x[i];
+            for (int u = 0; u < 4; ++u) wt[(4 * u + gq) * kTileStride + i] = wv[u];
         }
-        {
-            // the block's swaps on the columns right of the panel, 32 lanes
-            // (one row each) per column: block row i becomes old row
-            // r0 + sSrc[i] (A_kj, to LDS) and a displaced row p_i >= r0 + 32
-            // becomes old row r0 + sSrc[p_i - r0] (back to A).  In each
-            // column every load precedes every store.
-            int const i = lane & 31, grp = tid >> 5;
-            int const pi = sPiv[r0 + i];
-            bool const disp = pi >= r0 + kLuBs;
-            int const srcb = r0 + sSrc[i], srcd = disp ? r0 + sSrc[pi - r0] : 0;
-            int const ncol = npad - r0 - kLuBs;
-            constexpr int kU = 4, kGroups = kLuThreads / kLuBs;
-            for (int j0 = grp; j0 < ncol; j0 += kGroups * kU) {
-                double xb[kU], xd[kU];
+        __syncthreads();
+        if (w == 0) {
+            d4 t = {0.0, 0.0, 0.0, 0.0}, u = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    int const j = r0 + kLuBs + j0 + kGroups * u;
-                    bool const in = j0 + kGroups * u < ncol;
-                    xb[u] = in ? fget(srcb, j) : 0.0;
-                    xd[u] = in && disp ? fget(srcd, j) : 0.0;
-                }
+            for (int q = 0; q < 4; ++q) t = M::mma(op_read(sP[1][0], q, lane), op_read_t(sW[0], q, lane), t);
 #pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    int const jr = j0 + kGroups * u;  // column relative to the trailing block
-                    if (jr >= ncol) continue;
-                    sR[jr >> 4][i >> 4][(i & 15) * kTileStride + (jr & 15)] = xb[u];
-                    if (disp) fset(pi, r0 + kLuBs + jr, xd[u]);
-                }
-            }
+            for (int q = 0; q < 4; ++q) u = M::mma(-op_read(sW[2], q, lane), t[q], u);
+            acc_to_lds(sW[1], u, lane);
         }
         __syncthreads();
 
-        // ---- U_kj^T = A_kj^T L_kk^{-T}: to A (rows r0..) and to LDS as an
-        // A operand.  U^T's first 16 columns take the block row's first row
-        // tile only (L_kk^{-1} is lower triangular).
-        for (int jt = w; jt < mt; jt += kLuWaves) {
+        // ---- the columns right of the panel, and the right-hand sides: each
+        // wavefront owns whole column tiles (tile T0 + w, T0 + w + 4, ...; tile
+        // NT is X).  Per tile every load comes first, through the block's row
+        // permutation (row R now holds row r0 + sSrc[R - r0]): the block row
+        // A_kj as a B operand, the trailing tiles in the accumulator layout;
+        // U_kj = W A_kj leaves the MFMA in the layout of the trailing
+        // update's A operand (U_kj^T), A_ij^T -= U_kj^T L_ik^T (K = 32, L_ik
+        // from the panel tiles); then U_kj and the trailing tiles are stored.
+        // No row is stored before every load of the tile has returned (the
+        // stores depend on all of them), so the swaps need no separate pass.
+        int const ntile = mt + (rhs ? 1 : 0);
+        for (int jt = w; jt < ntile; jt += kLuWaves) {
+            int const tj = T0 + jt;
+            bool const isx = tj == NT;
+            Sep<T> const S = isx ? sx : sa;
+            int const c0 = isx ? 0 : 16 * tj, cmax = isx ? nrhs : n;
+            // every element outside n x n (and every right-hand-side column
+            // >= nrhs) reads as 0 here and is not stored: the padding takes no
+            // part in the real rows' and columns' results
+            int const cg = c0 + m, cpg = S.cols(min(cg, cmax - 1));
+            bool const vcg = cg < cmax;
+            double kv[2][4];
+            int kr[2][4];
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    kr[t][q] = r0 + sSrc[16 * t + 4 * q + gq];
+                    kv[t][q] = double(S.at(S.rows(min(kr[t][q], n - 1)) + cpg));
+                }
+            int cpr[4];
+            bool vcr[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int const c = c0 + gq + 4 * r;
+                vcr[r] = c < cmax;
+                cpr[r] = S.cols(min(c, cmax - 1));
+            }
+            // trailing tiles, kLuGroup at a time (every load of a group in
+            // flight together; the other wavefronts of the CU, two instances'
+            // worth, run meanwhile); the padding outside n x n is replaced by
+            // 0 once the group has arrived.  A row
+            // below the block holds either its own content or a row of the
+            // block (a swap at step c exchanges position c with a later one),
+            // so no trailing store clobbers another trailing tile's source; the
+            // block's rows are overwritten by U_kj only after every load of
+            // the column tile.
+            auto load_group = [&](int i0, d4 (&dst)[kLuGroup], int (&rows)[kLuGroup]) __attribute__((always_inline)) {
+#pragma unroll
+                for (int u = 0; u < kLuGroup; ++u) {
+                    int const ti = T0 + i0 + u;
+                    int const R = ti < NT ? r0 + sSrc[16 * ti + m - r0] : n;
+                    int const rpR = S.rows(min(R, n - 1));
+                    rows[u] = R;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) dst[u][r] = double(S.at(rpR + cpr[r]));
+                }
+            };
+            d4 cur[kLuGroup];
+            int curR[kLuGroup];
+            if (mt > 0) load_group(0, cur, curR);
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) kv[t][q] = (vcg && kr[t][q] < n) ? pin(kv[t][q]) : 0.0;
             d4 u0 = {0.0, 0.0, 0.0, 0.0}, u1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                double const a0 = op_read(sR[jt][0], q, lane);
-                u0 = M::mma(a0, op_read(sW[0][0], q, lane), u0);
-                u1 = M::mma(a0, op_read(sW[1][0], q, lane), u1);
+                u0 = M::mma(op_read(sW[0], q, lane), kv[0][q], u0);
+                u1 = M::mma(op_read(sW[1], q, lane), kv[0][q], u1);
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) u1 = M::mma(op_read(sR[jt][1], q, lane), op_read(sW[1][1], q, lane), u1);
+            for (int q = 0; q < 4; ++q) u1 = M::mma(op_read(sW[2], q, lane), kv[1][q], u1);
+            for (int i0 = 0; i0 < mt; i0 += kLuGroup) {
+                if (i0 > 0) load_group(i0, cur, curR);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                fset(r0 + (lane & 15), 16 * (T0 + jt) + M::row(lane, r), u0[r]);
-                fset(r0 + 16 + (lane & 15), 16 * (T0 + jt) + M::row(lane, r), u1[r]);
-            }
-            acc_to_lds(sR[jt][0], u0, lane);
-            acc_to_lds(sR[jt][1], u1, lane);
-        }
-        __syncthreads();
-
-        // ---- trailing A_ij^T -= U_kj^T L_ik^T, i, j >= T0 (16-tiles), K = 32;
-        // wavefront w takes tiles w, w + 4, ..., kLuGroup at a time, the next
-        // group's loads issued before the current group's MFMAs
-        int const cnt = mt * mt;
-        auto tile_of = [&](int t, int& ti, int& tj) {
-            ti = t < cnt ? T0 + t % mt : -1;
-            tj = T0 + (t < cnt ? t / mt : 0);
-        };
-        auto load_group = [&](int t0, d4 (&acc)[kLuGroup]) {
+                for (int u = 0; u < kLuGroup; ++u)
 #pragma unroll
-            for (int u = 0; u < kLuGroup; ++u) {
-                int ti, tj;
-                tile_of(t0 + kLuWaves * u, ti, tj);
+                    for (int r = 0; r < 4; ++r) cur[u][r] = (curR[u] < n && vcr[r]) ? pin(cur[u][r]) : 0.0;
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    acc[u][r] = ti >= 0 ? fget(16 * ti + (lane & 15), 16 * tj + M::row(lane, r)) : 0.0;
-            }
-        };
-        constexpr int kStep = kLuWaves * kLuGroup;
-        d4 cur[kLuGroup], nxt[kLuGroup];
-        if (w < cnt) load_group(w, cur);
-        for (int t0 = w; t0 < cnt; t0 += kStep) {
-            if (t0 + kStep < cnt) load_group(t0 + kStep, nxt);
+                for (int u = 0; u < kLuGroup; ++u) {
+                    int const ti = T0 + i0 + u;
+                    if (ti >= NT) break;
+                    double const* const l0 = sP[ti - 2 * kb][0];
+                    double const* const l1 = sP[ti - 2 * kb][1];
 #pragma unroll
-            for (int u = 0; u < kLuGroup; ++u) {
-                int ti, tj;
-                tile_of(t0 + kLuWaves * u, ti, tj);
-                if (ti < 0) continue;
+                    for (int q = 0; q < 4; ++q) cur[u] = M::mma(-u0[q], op_read(l0, q, lane), cur[u]);
 #pragma unroll
-                for (int s2 = 0; s2 < 2; ++s2) {
-                    double const* const ut = sR[tj - T0][s2];
-                    double const* const lt = sP[ti - 2 * kb][s2];
+                    for (int q = 0; q < 4; ++q) cur[u] = M::mma(-u1[q], op_read(l1, q, lane), cur[u]);
+                    int const R = 16 * ti + m;
+                    if (R < n) {
+                        int const rpR = S.rows(R);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) cur[u] = M::mma(-op_read(ut, q, lane), op_read(lt, q, lane), cur[u]);
+                        for (int r = 0; r < 4; ++r)
+                            if (vcr[r]) S.at(rpR + cpr[r]) = T(cur[u][r]);
+                    }
                 }
-#pragma unroll
-                for (int r = 0; r < 4; ++r) fset(16 * ti + (lane & 15), 16 * tj + M::row(lane, r), cur[u][r]);
             }
-            if (t0 + kStep < cnt)
+            // every load of this column tile has returned before the block's
+            // rows are overwritten
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            if (vcg)
 #pragma unroll
-                for (int u = 0; u < kLuGroup; ++u) cur[u] = nxt[u];
+                for (int r = 0; r < 4; ++r) {
+                    int const R0 = r0 + gq + 4 * r, R1 = R0 + 16;
+                    if (R0 < n) S.at(S.rows(R0) + cpg) = T(u0[r]);
+                    if (R1 < n) S.at(S.rows(R1) + cpg) = T(u1[r]);
+                }
         }
         __syncthreads();
     }
@@ -350,54 +484,35 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs<T> g) {
 #pragma unroll
                 for (int c = 0; c < kLuBs; ++c) fset(tid, kLuBs * (kb0 - u) + c, t[u][c]);
     }
-#pragma unroll
-    for (int s = kLuBs - 1; s >= 0; --s) {  // block 0's swaps: sig = the source row of row tid for B
-        int const p = sPiv[s];
-        sig = sig == s ? p : (sig == p ? s : sig);
-    }
     __syncthreads();  // the solve's LDS overlays the factor's
 
     int const bad = sFlag;
     if (tid == 0 && g.info) g.info[b] = bad;
-    if (bad || nrhs == 0) return;
+    if (!rhs) return;
 
-    // ---- solve
-    auto& sT = sm.s.sT;
+    // ---- U x = y (X holds y): the diagonal blocks' inverses U_kk^{-1}
+    // (tri_inverse, one wavefront per block), then block back substitution on
+    // MFMA.  A zero pivot (info > 0) gives inf / NaN in X.
+    auto& sInv = sm.s.sInv;
     auto& sX = sm.s.sX;
-    Ref<T> bb(g.B, b);
     if (tid < npad)
 #pragma unroll
-        for (int c = 0; c < kLuRhs; ++c) sX[tid * kLuRhs + c] = (tid < n && c < nrhs) ? double(bb(sig, c)) : 0.0;
-    for (int e = tid; e < NT * 256; e += kLuThreads) {  // the diagonal blocks
-        int const k = e >> 8, c = (e >> 4) & 15, m = e & 15;
-        sT[2][k][c * kTileStride + m] = fget(16 * k + m, 16 * k + c);
-    }
-    __syncthreads();
-    for (int task = tid; task < 2 * NT * 16; task += kLuThreads) {
-        int const k = (task >> 4) % NT, cc = task & 15;
-        double const* const d = sT[2][k];
-        double x[16];
-        if (task < NT * 16) {  // column cc of the unit-lower inverse
+        for (int c = 0; c < kLuRhs; ++c) sX[tid * kLuRhs + c] = xget(tid, c);
+    for (int k = w; k < NT; k += kLuWaves) {
+        // U_kk^T is lower triangular: lane (g, i) forms row i of (U_kk^T)^{-1},
+        // i.e. column i of U_kk^{-1}, from column i of U_kk; every read of the
+        // tile precedes the wavefront's writes
+        double* const t = sInv[k];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                double s = 0.0;
-#pragma unroll
-                for (int kk = 0; kk < i; ++kk) s = __builtin_fma(-d[kk * kTileStride + i], x[kk], s);
-                x[i] = (i == cc) ? 1.0 : (i < cc ? 0.0 : s);
-            }
-#pragma unroll
-            for (int i = 0; i < 16; ++i) sT[0][k][cc * kTileStride + i] = x[i];
-        } else {  // column cc of the upper inverse
-#pragma unroll
-            for (int i = 15; i >= 0; --i) {
-                double s = 0.0;
-#pragma unroll
-                for (int kk = i + 1; kk < 16; ++kk) s = __builtin_fma(-d[kk * kTileStride + i], x[kk], s);
-                x[i] = (i > cc) ? 0.0 : ((i == cc ? 1.0 : s) / d[i * kTileStride + i]);
-            }
-#pragma unroll
-            for (int i = 0; i < 16; ++i) sT[1][k][cc * kTileStride + i] = x[i];
+        for (int e = 0; e < 4; ++e) {
+            int const c = gq + 4 * e;
+            t[c * kTileStride + m] = fget(16 * k + m, 16 * k + c);
         }
+        double const myr = 1.0 / t[m * kTileStride + m];
+        double wv[4];
+        tri_inv16_regs(wv, lane, myr, [&](int j) { return j < m ? -t[m * kTileStride + j] : 0.0; }, NoHook{});
+#pragma unroll
+        for (int u = 0; u < 4; ++u) t[m * kTileStride + 4 * u + gq] = wv[u];
     }
     __syncthreads();
     // right-hand-side block as MFMA B operand (rows 16 blk + 4 q + l / 16,
@@ -417,37 +532,6 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs<T> g) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) sX[(16 * blk + M::row(lane, r)) * kLuRhs + rc] = v[r];
     };
-    for (int bk = 0; bk < NT; ++bk) {  // L y = P b
-        // the update's L tiles first: their loads are in flight during the
-        // diagonal step and the barrier
-        double fv[4][4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            int const i = bk + 1 + w + kLuWaves * u;
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                fv[u][q] = i < NT ? fget(16 * i + (lane & 15), 16 * bk + 4 * q + (lane >> 4)) : 0.0;
-        }
-        if (w == 0) {
-            d4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc = M::mma(op_read(sT[0][bk], q, lane), rhs_op(bk, q), acc);
-            rhs_store(bk, acc);
-        }
-        __syncthreads();
-        {  // y_i -= L_ib y_b, up to four tiles per wavefront
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                int const i = bk + 1 + w + kLuWaves * u;
-                if (i >= NT) continue;
-                d4 acc = rhs_load(i);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc = M::mma(-fv[u][q], rhs_op(bk, q), acc);
-                rhs_store(i, acc);
-            }
-        }
-        __syncthreads();
-    }
     for (int bk = NT - 1; bk >= 0; --bk) {  // U x = y
         double fv[4][4];
 #pragma unroll
@@ -460,7 +544,7 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs<T> g) {
         if (w == 0) {
             d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc = M::mma(op_read(sT[1][bk], q, lane), rhs_op(bk, q), acc);
+            for (int q = 0; q < 4; ++q) acc = M::mma(op_read(sInv[bk], q, lane), rhs_op(bk, q), acc);
             rhs_store(bk, acc);
         }
         __syncthreads();
@@ -484,7 +568,23 @@ __global__ __launch_bounds__(kLuThreads) void lu_wg_kernel(LuArgs<T> g) {
     }
 }
 
+template <typename T>
+bool offsets_fit(Mat<T> const& m, int rows, int cols) {
+    if (rows <= 0 || cols <= 0) return true;
+    constexpr int PS = PanelSize<T>::value, LOG = PanelSize<T>::log2;
+    int64_t const i = rows - 1, j = cols - 1, r = m.trans ? j : i, c = m.trans ? i : j;
+    int64_t const off = m.panel ? (r >> LOG) * m.spacing + (r & (PS - 1)) + c * PS : r + int64_t(m.spacing) * c;
+    return off >= 0 && off * int64_t(sizeof(T)) < (int64_t(1) << 31);
+}
+
 }  // namespace
+
+// The kernel addresses an instance with 32-bit byte offsets: true when every
+// element of A (and of X's first nrhs columns) is within 2 GiB of its base.
+bool lu_wg_fits(Mat<double> A, Mat<double> X, int nrhs) {
+    return offsets_fit(A, A.rows, A.cols) && (nrhs == 0 || offsets_fit(X, X.rows, nrhs));
+}
+bool lu_wg_fits_f32(Mat<float> A) { return offsets_fit(A, A.rows, A.cols); }
 
 // FP64 LU with partial pivoting in place of A, pivots to ipiv, and (nrhs > 0)
 // the solve into X; only_if: solve only the instances with only_if[b] < 0.

@@ -449,7 +449,7 @@ __device__ __attribute__((noinline)) int diag_factor(T* sD, T* sW, int lane) {
 // row i of the lower-triangular form, rows finalised at step j are
 // broadcast by DPP from lane j.
 template <typename T>
-__device__ void tri_inverse(const T* sA, T* sW, int lane, bool upper, bool unit) {
+__device__ __forceinline__ void tri_inverse_inl(const T* sA, T* sW, int lane, bool upper, bool unit) {
     int const i = lane & 15;
     T t[16];
 #pragma unroll
@@ -485,6 +485,10 @@ __device__ void tri_inverse(const T* sA, T* sW, int lane, bool upper, bool unit)
                 sW[c * kTileStride + i] = v;
         }
     }
+}
+template <typename T>
+__device__ void tri_inverse(const T* sA, T* sW, int lane, bool upper, bool unit) {
+    tri_inverse_inl(sA, sW, lane, upper, unit);
 }
 
 }  // namespace blast_hip
