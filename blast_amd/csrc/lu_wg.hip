@@ -91,6 +91,33 @@ __device__ __forceinline__ void wave_argmax(double& v, int& i) {
     argmax_swap<false>(v, i);
 }
 
+// Wavefront maximum of an FP64 value without NaNs, in every lane: four DPP
+// steps within each 16-lane row, then the two row swaps.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    unsigned long long const u = __builtin_bit_cast(unsigned long long, v);
+    unsigned const lo = unsigned(__builtin_amdgcn_mov_dpp(int(unsigned(u)), CTRL, 0xF, 0xF, false));
+    unsigned const hi = unsigned(__builtin_amdgcn_mov_dpp(int(unsigned(u >> 32)), CTRL, 0xF, 0xF, false));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+template <bool ROW16>
+__device__ __forceinline__ double swap_max(double v) {
+    unsigned long long const u = __builtin_bit_cast(unsigned long long, v);
+    unsigned l0, l1, h0, h1;
+    lane_swap<ROW16>(unsigned(u), l0, l1);
+    lane_swap<ROW16>(unsigned(u >> 32), h0, h1);
+    return __builtin_fmax(__builtin_bit_cast(double, (static_cast<unsigned long long>(h0) << 32) | l0),
+                          __builtin_bit_cast(double, (static_cast<unsigned long long>(h1) << 32) | l1));
+}
+__device__ __forceinline__ double wave_max(double v) {
+    v = __builtin_fmax(v, dpp_f64<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = __builtin_fmax(v, dpp_f64<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = __builtin_fmax(v, dpp_f64<0x141>(v));  // row_half_mirror
+    v = __builtin_fmax(v, dpp_f64<0x140>(v));  // row_mirror
+    v = swap_max<true>(v);
+    return swap_max<false>(v);
+}
+
 // T: the storage type (the arithmetic is FP64 for FP32 storage too: values
 // are widened on load and rounded on store).
 template <typename T>
@@ -251,23 +278,42 @@ __global__ __launch_bounds__(kLuThreads, 2) void lu_wg_kernel(LuArgs<T> g) {
             constexpr bool kMasked = decltype(MASKED)::value;
             static_for<0, kLuBs>([&](auto C) __attribute__((always_inline)) {
                 constexpr int c = decltype(C)::value, buf = c & 1;
+                // The pivot is the candidate (position >= c) with the largest
+                // key, the lowest position among equal keys: the first maximum
+                // of BLAST's scan from position c with a strict '>'.  Keys:
+                // |v|; a NaN at position c (the scan's first element, never
+                // replaced) +inf; a NaN elsewhere -1 (never chosen); no
+                // candidate -2.  So the keys hold no NaN, and the search is a
+                // wavefront maximum plus a ballot for the lowest position
+                // holding it (the full (key, position) reduction only on a
+                // tie).
                 bool const cand = live && pos >= c;
-                double kv = cand ? iamax_key(v[c], pos - c) : -1.0;
-                int ki = cand ? pos : 0x7fffffff;
-                wave_argmax(kv, ki);
-                if (pos == ki)
+                double const av = __builtin_fabs(v[c]);
+                double const key = !cand ? -2.0 : (av != av ? (pos == c ? __builtin_inf() : -1.0) : av);
+                double const wm = wave_max(key);
+                int wpos = 0x7fffffff;
+                if (wm > -2.0) {
+                    unsigned long long const tie = __ballot(key == wm);
+                    if (__popcll(tie) == 1) {
+                        wpos = __builtin_amdgcn_readlane(pos, int(__builtin_ctzll(tie)));
+                    } else {
+                        double kv = key;
+                        int ki = cand ? pos : 0x7fffffff;
+                        wave_argmax(kv, ki);
+                        wpos = ki;
+                    }
+                }
+                if (pos == wpos)
 #pragma unroll
                     for (int cc = c; cc < kLuBs; ++cc) sRow[buf][w][cc] = v[cc];
-                if (lane == 0) sCandV[buf][w] = kv, sCandI[buf][w] = ki;
+                if (lane == 0) sCandV[buf][w] = wm, sCandI[buf][w] = wpos;
                 __syncthreads();
-                double pv = sCandV[buf][0];
-                int pp = sCandI[buf][0], wp = 0;
-#pragma unroll
-                for (int q = 1; q < kLuWaves; ++q) {
-                    int const before = pp;
-                    argmax_combine(pv, pp, sCandV[buf][q], sCandI[buf][q]);
-                    wp = pp != before ? q : wp;
-                }
+                double const k0 = sCandV[buf][0], k1 = sCandV[buf][1], k2 = sCandV[buf][2], k3 = sCandV[buf][3];
+                int const p0 = sCandI[buf][0], p1 = sCandI[buf][1], p2 = sCandI[buf][2], p3 = sCandI[buf][3];
+                double const km = __builtin_fmax(__builtin_fmax(k0, k1), __builtin_fmax(k2, k3));
+                int const pp = min(min(k0 == km ? p0 : 0x7fffffff, k1 == km ? p1 : 0x7fffffff),
+                                   min(k2 == km ? p2 : 0x7fffffff, k3 == km ? p3 : 0x7fffffff));
+                int const wp = p1 == pp ? 1 : (p2 == pp ? 2 : (p3 == pp ? 3 : 0));
                 // position c always holds a candidate, so pp is a real position
                 pos = pos == pp ? c : (pos == c ? pp : pos);
                 if (tid == 0) sPiv[r0 + c] = r0 + pp;
