@@ -396,8 +396,38 @@ __global__ __launch_bounds__(kLuThreads, 2) void lu_wg_kernel(LuArgs<T> g) {
         // from the panel tiles); then U_kj and the trailing tiles are stored.
         // No row is stored before every load of the tile has returned (the
         // stores depend on all of them), so the swaps need no separate pass.
-        int const ntile = mt + (rhs ? 1 : 0);
-        for (int jt = w; jt < ntile; jt += kLuWaves) {
+        // Then the tiles left of the panel (columns < r0, the L of earlier
+        // blocks) take the block's swaps: rows r0.. of the block receive rows
+        // r0 + sSrc[i], and each row below that a swap displaced (position p
+        // = ipiv - r0 >= 32) receives row r0 + sSrc[p], a row of the block.
+        int const ntile = mt + (rhs ? 1 : 0), nleft = 2 * kb;
+        for (int jt = w; jt < ntile + nleft; jt += kLuWaves) {
+            if (jt >= ntile) {
+                int const col = 16 * (jt - ntile) + m, cpc = sa.cols(col);
+                double bv[2][4], dv[2][4];
+                int drow[2][4];
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        int const i = 16 * t + 4 * q + gq, pi = sPiv[r0 + i];
+                        int const src = r0 + sSrc[i];
+                        drow[t][q] = pi >= r0 + kLuBs ? pi : n;
+                        int const dsrc = pi >= r0 + kLuBs ? r0 + sSrc[pi - r0] : src;
+                        bv[t][q] = double(sa.at(sa.rows(min(src, n - 1)) + cpc));
+                        dv[t][q] = double(sa.at(sa.rows(min(dsrc, n - 1)) + cpc));
+                    }
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): every row read before any is written
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        int const R = r0 + 16 * t + 4 * q + gq;
+                        if (R < n) sa.at(sa.rows(R) + cpc) = T(bv[t][q]);
+                        if (drow[t][q] < n) sa.at(sa.rows(drow[t][q]) + cpc) = T(dv[t][q]);
+                    }
+                continue;
+            }
             int const tj = T0 + jt;
             bool const isx = tj == NT;
             Sep<T> const S = isx ? sx : sa;
@@ -498,38 +528,6 @@ __global__ __launch_bounds__(kLuThreads, 2) void lu_wg_kernel(LuArgs<T> g) {
         __syncthreads();
     }
 
-    // ---- the columns left of each panel get the later blocks' swaps: thread
-    // r traces its row back through them (last swap first) to the row whose
-    // content ends up in row r; sig accumulates from the last block down.
-    // Blocks are handled kChunk at a time: their gathers, one barrier, their
-    // stores (different blocks touch different columns).
-    int sig = tid;
-    constexpr int kChunk = 2;
-    for (int kb0 = NB - 2; kb0 >= 0; kb0 -= kChunk) {
-        double t[kChunk][kLuBs];
-        bool mv[kChunk];
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-            int const kb = kb0 - u;
-            if (kb >= 0) {
-                int const s0 = kLuBs * (kb + 1);
-#pragma unroll
-                for (int s = kLuBs - 1; s >= 0; --s) {
-                    int const rr = s0 + s, p = sPiv[rr];
-                    sig = sig == rr ? p : (sig == p ? rr : sig);
-                }
-            }
-            mv[u] = kb >= 0 && tid < npad && sig != tid;
-#pragma unroll
-            for (int c = 0; c < kLuBs; ++c) t[u][c] = mv[u] ? fget(sig, kLuBs * kb + c) : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u)
-            if (mv[u])
-#pragma unroll
-                for (int c = 0; c < kLuBs; ++c) fset(tid, kLuBs * (kb0 - u) + c, t[u][c]);
-    }
     __syncthreads();  // the solve's LDS overlays the factor's
 
     int const bad = sFlag;
@@ -578,15 +576,21 @@ __global__ __launch_bounds__(kLuThreads, 2) void lu_wg_kernel(LuArgs<T> g) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) sX[(16 * blk + M::row(lane, r)) * kLuRhs + rc] = v[r];
     };
-    for (int bk = NT - 1; bk >= 0; --bk) {  // U x = y
-        double fv[4][4];
+    // U x = y by 16-row blocks from the bottom; the U tiles of the next
+    // block are loaded while this one is solved
+    double fv[4][4], fn[4][4];
+    auto load_u = [&](int bk, double (&dst)[4][4]) __attribute__((always_inline)) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             int const i = w + kLuWaves * u;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                fv[u][q] = i < bk ? fget(16 * i + (lane & 15), 16 * bk + 4 * q + (lane >> 4)) : 0.0;
+                dst[u][q] = i < bk ? fget(16 * i + (lane & 15), 16 * bk + 4 * q + (lane >> 4)) : 0.0;
         }
+    };
+    load_u(NT - 1, fv);
+    for (int bk = NT - 1; bk >= 0; --bk) {
+        if (bk > 0) load_u(bk - 1, fn);
         if (w == 0) {
             d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -606,6 +610,10 @@ __global__ __launch_bounds__(kLuThreads, 2) void lu_wg_kernel(LuArgs<T> g) {
             }
         }
         __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) fv[u][q] = fn[u][q];
     }
     Ref<T> x(g.X, b);
     for (int e = tid; e < n * nrhs; e += kLuThreads) {
