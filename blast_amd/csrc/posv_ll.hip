@@ -15,7 +15,8 @@
 // v_mfma_f64_16x16x4f64 produces when it computes M^T.  So every product of
 // the factorisation is formed transposed and nothing passes through LDS:
 //   update   A_ij^T -= L_jk L_ik^T            (k < j <= i)
-//   factor   the diagonal tile in place in the same layout (chol16_regs)
+//   factor   the diagonal tile in the same layout, each column of L_jj
+//            through LDS (chol16_cols)
 //   inverse  W_j = L_jj^{-1} (tri_inv16_regs), in the same layout
 //   panel    L_ij^T = W_j A_ij^T              (i > j)
 //   forward  r_j = b_j - sum_k L_jk y_k, y_j = W_j r_j
@@ -218,15 +219,11 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
         // forward: r_j = b_j - sum_k L_jk y_k (result layout)
         double r = 0.0;
         if (nrhs > 0) r = sw[yd];
-        // the diagonal factorisation (one tile row per lane, through LDS:
-        // b_j has been read), with the column's other updates and the
-        // forward update spread over its 16 steps
-#pragma unroll
-        for (int q = 0; q < 4; ++q) sw[(4 * q + g) * kTileStride + m] = t[tri(j, j)][q];
-        double row[16];
-#pragma unroll
-        for (int c = 0; c < 16; ++c) row[c] = sw[c * kTileStride + m];
-        chol16_rows(row, [&](auto S) __attribute__((always_inline)) {
+        // the diagonal factorisation in the transposed layout, each column
+        // of L_jj through the LDS tile (b_j has been read: its slots are
+        // free until r is stored back), with the column's other updates and
+        // the forward update spread over its 16 steps
+        chol16_cols(t[tri(j, j)], sw, lane, [&](auto S) __attribute__((always_inline)) {
             constexpr int s = decltype(S)::value;
             static_for<Ops::ops * s / 16, Ops::ops * (s + 1) / 16>([&](auto O) __attribute__((always_inline)) {
                 constexpr int e = Ops::encode(decltype(O)::value);
@@ -248,9 +245,6 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
                 }
             });
         });
-        if (g == 0)
-#pragma unroll
-            for (int c = 0; c < 16; ++c) sw[c * kTileStride + m] = row[c];
         double dt[4];  // L_jj in the transposed layout (L[m][4u + g])
 #pragma unroll
         for (int u = 0; u < 4; ++u) dt[u] = sw[(4 * u + g) * kTileStride + m];

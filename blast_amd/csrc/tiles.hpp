@@ -348,6 +348,43 @@ __device__ __forceinline__ void chol16_rows(double (&row)[16], Hook&& hook) {
     });
 }
 
+// chol16_cols: the same factorisation on the tile in the transposed register
+// layout t (lane (g, m) holds row m in the columns 4u + g, t[u] = A[m][4u + g],
+// the layout the MFMA updates leave it in), each column passing through LDS:
+// column c of the 16x16 LDS tile sw (element (m, c) at c*17 + m) receives
+// column c of L.  At step k the pivot a_kk is broadcast from lane k of each
+// 16-lane row (DPP; the owners of column k, group k % 4, receive a_kk),
+// r = 1/sqrt(a_kk), the owners of column k write
+// l_mk = a_mk r to LDS, and every lane reads l_mk and the l_ck of its four
+// columns c = 4u + g and updates a_mc -= l_mk l_ck: four FMAs per step
+// instead of a broadcast and an FMA per trailing column.  The per-element
+// operations are chol16_rows' (same FMAs in the same order).  Entries of
+// finished columns, and rows above the pivot, receive values that are never
+// read again: the factor is the one in LDS (its strict upper triangle holds
+// garbage and is overwritten by the caller; rows below the diagonal of each
+// column are exact).  hook(integral_constant<int, k>) runs after column k is
+// written.
+template <typename Hook>
+__device__ __forceinline__ void chol16_cols(d4& t, double* sw, int lane, Hook&& hook) {
+    int const m = lane & 15, g = lane >> 4;
+    static_for<0, 16>([&](auto K) __attribute__((always_inline)) {
+        constexpr int k = decltype(K)::value, kq = k >> 2, kg = k & 3;
+        double const r = rsqrt_nr(row_bcast<k>(t[kq]));  // a_kk in the lanes of group k % 4
+        double const lk = t[kq] * r;
+        if (g == kg) sw[k * kTileStride + m] = lk;
+        // the other lanes read what group k % 4 wrote: without this fence the
+        // compiler, reasoning per lane, lets a lane that did not store load
+        // before the store (LDS itself keeps a wavefront's accesses in order)
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        hook(K);
+        double const lmk = sw[k * kTileStride + m];
+        static_for<0, 4>([&](auto U) __attribute__((always_inline)) {
+            constexpr int u = decltype(U)::value;
+            if constexpr (4 * u + 3 > k) t[u] = fma_t(-lmk, sw[k * kTileStride + 4 * u + g], t[u]);
+        });
+    });
+}
+
 // tri_inv16_regs: W = L^{-1} of the 16x16 lower-triangular tile whose
 // multipliers are read by mult(j) = -l_ij (row i = lane % 16; 0 for j >= i),
 // with myr = 1/l_ii of the lane's row.  Lane (g, i) computes W[i][c] for its
