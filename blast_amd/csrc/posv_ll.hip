@@ -95,23 +95,28 @@ __device__ __forceinline__ d4 load_acc(const double* p, int spacing, int ti, int
 //           for i = j+1..NT-1 (tile (j, j-1) was formed at the end of step
 //           j-1), four K-chunks per tile;
 //   kind 1: y_j-1 = W_j-1 r_j-1, four K-chunks;
+//   kind 3: the next diagonal tile's update A_j+1,j+1 -= L_j+1,k L_j+1,k^T
+//           for k < j (its k = j term opens step j+1), k outer, q inner;
 //   kind 2: for k < j (outer), K-chunk q, then row i = j+1..NT-1 (the update
 //           A_ij^T -= L_jk L_ik^T) and i = NT (the forward update
 //           r_j -= L_jk y_k), so that consecutive MFMAs accumulate into
 //           different tiles.
-// The k = j-1 terms come last, after the panel tiles and y_j-1 they read.
-// Encoded as (kind << 24) | (k << 16) | (q << 8) | i.
+// The k = j-1 terms of kinds 3 and 2 come after the panel tiles and y_j-1
+// they read.  Encoded as (kind << 24) | (k << 16) | (q << 8) | i.
 template <int NT, int J>
 struct StepOps {
     static constexpr int panel = J >= 1 ? 4 * (NT - 1 - J) : 0;
     static constexpr int yops = J >= 1 ? 4 : 0;
+    static constexpr int nextdiag = J + 1 < NT ? 4 * J : 0;
     static constexpr int rows = NT - J;  // NT - 1 - J tiles and the right-hand sides
-    static constexpr int ops = panel + yops + J * 4 * rows;
+    static constexpr int ops = panel + yops + nextdiag + J * 4 * rows;
     static constexpr int encode(int o) {
         if (o < panel) return (0 << 24) | ((o % 4) << 8) | (J + 1 + o / 4);
         o -= panel;
         if (o < yops) return (1 << 24) | (o << 8);
         o -= yops;
+        if (o < nextdiag) return (3 << 24) | ((o / 4) << 16) | ((o % 4) << 8) | (J + 1);
+        o -= nextdiag;
         int const k = o / (4 * rows), q = (o / rows) % 4, i = J + 1 + o % rows;
         return (2 << 24) | (k << 16) | (q << 8) | i;
     }
@@ -137,14 +142,18 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
     // t[tri(i, j)]: tile (i, j) of A while column j is updated, then L_ij;
     // transposed layout.
     d4 t[NTILES];
+    // column j's tiles below the diagonal and the next diagonal tile (j+1,
+    // j+1), whose update starts during step j (column j's own diagonal tile
+    // came with column j-1)
     auto load_col = [&](auto J) __attribute__((always_inline)) {
         constexpr int j = decltype(J)::value;
-        t[tri(j, j)] = load_t_diag(pa, sa, j, lane);
+        if constexpr (j + 1 < NT) t[tri(j + 1, j + 1)] = load_t_diag(pa, sa, j + 1, lane);
         static_for<j + 1, NT>([&](auto I) __attribute__((always_inline)) {
             constexpr int i = decltype(I)::value;
             t[tri(i, j)] = load_t(pa, sa, loa, i, j);
         });
     };
+    t[tri(0, 0)] = load_t_diag(pa, sa, 0, lane);
     load_col(std::integral_constant<int, 0>{});
     if (nrhs > 0) {  // every load issued before the first LDS store
         Ref<double> bb(B, b);
@@ -210,12 +219,13 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
                 load_row(std::integral_constant<int, NT - 1>{});
             }
         }
-        // diagonal tile: A_jj -= sum_k L_jk L_jk^T (whole tile, symmetric)
-        static_for<0, j>([&](auto K) __attribute__((always_inline)) {
-            constexpr int k = decltype(K)::value;
+        // diagonal tile: A_jj -= L_j,j-1 L_j,j-1^T (whole tile, symmetric; the
+        // terms k < j-1 were applied during step j-1)
+        if constexpr (j >= 1) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) t[tri(j, j)] = M::mma(-t[tri(j, k)][q], t[tri(j, k)][q], t[tri(j, j)]);
-        });
+            for (int q = 0; q < 4; ++q)
+                t[tri(j, j)] = M::mma(-t[tri(j, j - 1)][q], t[tri(j, j - 1)][q], t[tri(j, j)]);
+        }
         // forward: r_j = b_j - sum_k L_jk y_k (result layout)
         double r = 0.0;
         if (nrhs > 0) r = sw[yd];
@@ -234,6 +244,8 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
                         t[tri(i, j - 1)] = pz;
                         store_l(i, j - 1, pz);
                     }
+                } else if constexpr (kind == 3) {  // next diagonal tile, term k
+                    t[tri(j + 1, j + 1)] = M::mma(-t[tri(j + 1, k)][q], t[tri(j + 1, k)][q], t[tri(j + 1, j + 1)]);
                 } else if constexpr (kind == 1) {  // y_j-1 = W_j-1 r_j-1 (r_j-1 in LDS)
                     if constexpr (q < 2) ypz[q] = mma4(wp[q], rhs_b(j - 1, q), 0.0);
                     else ypz[q - 2] = mma4(wp[q], rhs_b(j - 1, q), ypz[q - 2]);
