@@ -53,6 +53,7 @@ __device__ __forceinline__ int ys(int rr, int c) {
 template <int NT>
 struct LlLds {
     double sW[NT][kLlTile];
+    double sR[16];  // r_k = 1/sqrt(pivot k) of the diagonal tile being factored
 };
 
 // Panel storage (PS = 4) offsets of the transposed register layout:
@@ -233,7 +234,8 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
         // of L_jj through the LDS tile (b_j has been read: its slots are
         // free until r is stored back), with the column's other updates and
         // the forward update spread over its 16 steps
-        chol16_cols(t[tri(j, j)], sw, lane, [&](auto S) __attribute__((always_inline)) {
+        double z[4];  // the inverse's running rows, W_j = -z r (chol16_cols_inv)
+        chol16_cols_inv(t[tri(j, j)], sw, sm.sR, lane, z, [&](auto S) __attribute__((always_inline)) {
             constexpr int s = decltype(S)::value;
             static_for<Ops::ops * s / 16, Ops::ops * (s + 1) / 16>([&](auto O) __attribute__((always_inline)) {
                 constexpr int e = Ops::encode(decltype(O)::value);
@@ -275,22 +277,17 @@ __global__ __launch_bounds__(64, 2) void posv_ll_kernel(Mat<double> A, Mat<doubl
                 if (c <= m || (c >> 2) == (m >> 2)) q0[16 * u] = c <= m ? dt[u] : 0.0;
             }
         }
-        // W_j = L_jj^{-1}: the multipliers -l_ij parked in the tile, l_ii read
-        // back from its diagonal and replaced by 0
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            int const c = 4 * u + g;
-            sw[c * kTileStride + m] = c < m ? -dt[u] : (c == m ? dt[u] : 0.0);
-        }
+        // W_j = L_jj^{-1} = -z r (formed with the factorisation), l_ii from
+        // the diagonal of the LDS factor for the pivot check
         double const lii = sw[m * kTileStride + m];
-        sw[m * kTileStride + m] = 0.0;
         int const bad = first_bad_pivot(lii, lane);
-        if (bad && inf == 0) inf = 16 * j + bad;
+        // (settled now: left to the end, the eight steps' ballots stay live
+        // in SGPR pairs and spill)
+        inf = opaque_s(bad && inf == 0 ? 16 * j + bad : inf);
+        double const rm = sm.sR[m];
         double w[4];
-        double rc = __builtin_amdgcn_rcp(lii);  // 1 / l_ii: estimate and two Newton steps
-        rc = fma_t(rc, fma_t(-lii, rc, 1.0), rc);
-        rc = fma_t(rc, fma_t(-lii, rc, 1.0), rc);
-        tri_inv16_regs(w, opaque_v(lane), rc, [&](int jj) { return sw[jj * kTileStride + m]; }, NoHook{});
+#pragma unroll
+        for (int u = 0; u < 4; ++u) w[u] = (4 * u + g <= m) ? -z[u] * rm : 0.0;
 #pragma unroll
         for (int u = 0; u < 4; ++u)  // lower triangle, for the backward solve
             if (4 * u + g <= m) sw[(4 * u + g) * kTileStride + m] = w[u];

@@ -385,6 +385,42 @@ __device__ __forceinline__ void chol16_cols(d4& t, double* sw, int lane, Hook&& 
     });
 }
 
+// chol16_cols_inv: chol16_cols fused with the inverse W = L^{-1}, whose step
+// k needs only column k of L: the two dependent chains interleave.  The
+// owners of column k also write r_k = 1/sqrt(a_kk) (= 1/l_kk to an ulp or
+// two) to sr[k].  Inverse (as tri_inv16_regs): lane (g, i) keeps z_i[c] for
+// c = 4u + g, z_i[c] = -delta_ic at first; at step k row k is final and rows
+// i > k apply it, z_i += (-l_ik r_k) z_k with z_k broadcast from lane k of the
+// lane's 16-lane row.  On exit W[i][c] = -z_i[c] r_i (the caller scales).
+template <typename Hook>
+__device__ __forceinline__ void chol16_cols_inv(d4& t, double* sw, double* sr, int lane, double (&z)[4],
+                                                Hook&& hook) {
+    int const m = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) z[u] = (4 * u + g == m) ? -1.0 : 0.0;
+    static_for<0, 16>([&](auto K) __attribute__((always_inline)) {
+        constexpr int k = decltype(K)::value, kq = k >> 2, kg = k & 3;
+        double const r = rsqrt_nr(row_bcast<k>(t[kq]));  // a_kk in the lanes of group k % 4
+        double const lk = t[kq] * r;
+        if (g == kg) {
+            sw[k * kTileStride + m] = lk;
+            sr[k] = r;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");  // as in chol16_cols
+        hook(K);
+        double const lmk = sw[k * kTileStride + m];
+        // rows below the pivot only: a sign mask, not a per-step compare (16
+        // of those kept in SGPR pairs spill)
+        unsigned long long const below = static_cast<unsigned long long>(static_cast<long long>((k - m) >> 31));
+        double const f = __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, -lmk * sr[k]) & below);
+        static_for<0, 4>([&](auto U) __attribute__((always_inline)) {
+            constexpr int u = decltype(U)::value;
+            if constexpr (4 * u <= k) z[u] = fma_t(f, row_bcast<k>(z[u]), z[u]);
+            if constexpr (4 * u + 3 > k) t[u] = fma_t(-lmk, sw[k * kTileStride + 4 * u + g], t[u]);
+        });
+    });
+}
+
 // tri_inv16_regs: W = L^{-1} of the 16x16 lower-triangular tile whose
 // multipliers are read by mult(j) = -l_ij (row i = lane % 16; 0 for j >= i),
 // with myr = 1/l_ii of the lane's row.  Lane (g, i) computes W[i][c] for its
