@@ -107,14 +107,104 @@ __device__ void wg_gemm(double alpha, DV A, DV B, double beta, CV C, DV D, int M
     __syncthreads();
 }
 
+// The same product with both operands staged through LDS: the workgroup
+// forms 64 x 16 tn blocks of D (wavefront w: rows 16 w..16 w + 15, tn = 1, 2
+// or 4 column tiles, a runtime choice so that one instantiation serves every
+// size bucket), K in chunks of 16 held in two LDS buffers (each element of a chunk
+// loaded once from memory and read by all four wavefronts, the next chunk's
+// loads in flight during the current chunk's MFMAs).  Loads follow each
+// operand's contiguous direction (DV::t).  kSyrkAdd skips the blocks above
+// the diagonal; kUpperA starts K at the block's first row.  sg: 4 * 16 * 65
+// doubles of LDS.
+constexpr int kGemmKc = 16, kGemmLs = 65, kGemmLds = 4 * kGemmKc * kGemmLs;
+template <int MODE, class CV = DV>
+__device__ void wg_gemm_lds(double alpha, DV A, DV B, double beta, CV C, DV D, int M, int N, int K, int tn,
+                            double* sg) {
+    using Mf = Mfma<double>;
+    constexpr int BM = 64, KC = kGemmKc, LS = kGemmLs, TN = 4;
+    int const lbn = tn == 4 ? 6 : (tn == 2 ? 5 : 4), BN = 1 << lbn;
+    int const tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int const m = lane & 15, g = lane >> 4;
+    int const MB = (M + BM - 1) / BM, NB = (N + BN - 1) >> lbn;
+    double av[4], bv[TN];
+    auto load_chunk = [&](int bi, int bj, int kc) __attribute__((always_inline)) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int const r = A.t ? tid / KC + 16 * i : tid % BM, kk = A.t ? tid % KC : tid / BM + 4 * i;
+            int const row = bi * BM + r, k = kc + kk;
+            bool const in = row < M && k < K && (MODE != kUpperA || k >= row);
+            av[i] = in ? A(row, k) : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < TN; ++i) {
+            if (i >= tn) break;
+            int const c = B.t ? tid & (BN - 1) : tid / KC + 16 * i;
+            int const kk = B.t ? (tid >> lbn) + (256 >> lbn) * i : tid % KC;
+            int const col = bj * BN + c, k = kc + kk;
+            bv[i] = (col < N && k < K) ? B(k, col) : 0.0;
+        }
+    };
+    auto store_chunk = [&](int buf) __attribute__((always_inline)) {
+        double* const sa = sg + (2 * buf) * KC * LS;
+        double* const sb = sa + KC * LS;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int const r = A.t ? tid / KC + 16 * i : tid % BM, kk = A.t ? tid % KC : tid / BM + 4 * i;
+            sa[kk * LS + r] = av[i];
+        }
+#pragma unroll
+        for (int i = 0; i < TN; ++i) {
+            if (i >= tn) break;
+            int const c = B.t ? tid & (BN - 1) : tid / KC + 16 * i;
+            int const kk = B.t ? (tid >> lbn) + (256 >> lbn) * i : tid % KC;
+            sb[kk * LS + c] = bv[i];
+        }
+    };
+    for (int bi = 0; bi < MB; ++bi) {
+        for (int bj = 0; bj < NB; ++bj) {
+            if (MODE == kSyrkAdd && bj * BN > bi * BM + BM - 1) break;
+            // an upper-triangular A has no entries left of its rows' diagonal
+            int const k0 = MODE == kUpperA ? (bi * BM / KC) * KC : 0;
+            d4 acc[TN];
+#pragma unroll
+            for (int t = 0; t < TN; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+            load_chunk(bi, bj, k0);
+            int buf = 0;
+            for (int kc = k0; kc < K; kc += KC) {
+                store_chunk(buf);
+                __syncthreads();
+                if (kc + KC < K) load_chunk(bi, bj, kc + KC);
+                double const* const sa = sg + (2 * buf) * KC * LS;
+                double const* const sb = sa + KC * LS;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    double const a = sa[(4 * q + g) * LS + 16 * w + m];
+#pragma unroll
+                    for (int t = 0; t < TN; ++t)
+                        if (t < tn) acc[t] = Mf::mma(a, sb[(4 * q + g) * LS + 16 * t + m], acc[t]);
+                }
+                buf ^= 1;
+            }
+#pragma unroll
+            for (int t = 0; t < TN; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    int const i = bi * BM + 16 * w + Mf::row(lane, r), j = bj * BN + 16 * t + acc_col(lane);
+                    if (t < tn && i < M && j < N && (MODE != kSyrkAdd || i >= j))
+                        D(i, j) = beta == 0.0 ? alpha * acc[t][r] : __builtin_fma(beta, C(i, j), alpha * acc[t][r]);
+                }
+            __syncthreads();  // the next block's first chunk reuses buffer 0
+        }
+    }
+}
+
 // In-place lower Cholesky of the n x n matrix a (lower triangle read and
 // written), blocked by 16 columns: wavefront 0 factors the 16x16 diagonal
 // block in LDS and forms W = L_kk^{-1} (diag_factor, no barriers inside), the
 // panel below is L_ik = A_ik W^T on MFMA, the trailing matrix is updated on
 // MFMA tiles.  Returns the first non-positive pivot column (1-based) or 0
 // (uniform across the workgroup).
-template <int TN>
-__device__ int wg_potrf(DV a, int n, double* sB, double* sW, int* sBad) {
+__device__ int wg_potrf(DV a, int n, int tn, double* sB, double* sW, int* sBad, double* sG) {
     using Mf = Mfma<double>;
     int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (threadIdx.x == 0) *sBad = 0;
@@ -166,8 +256,13 @@ __device__ int wg_potrf(DV a, int n, double* sB, double* sW, int* sBad) {
         if (rest > 0) {
             int const gi = (k0 >> 4) & 3;
             DV const l21 = a.sub(r0, k0 - 16 * gi), a22 = a.sub(r0, r0);
-            wg_gemm<TN, kSyrkAdd>(-1.0, l21, l21.tr(), 1.0, a22, a22, rest, gi == 3 ? rest : min(16, rest),
-                                  16 * gi + kb);
+            int const nn = gi == 3 ? rest : min(16, rest), kk = 16 * gi + kb;
+            if (tn == 1)
+                wg_gemm<1, kSyrkAdd>(-1.0, l21, l21.tr(), 1.0, a22, a22, rest, nn, kk);
+            else if (tn == 2)
+                wg_gemm<2, kSyrkAdd>(-1.0, l21, l21.tr(), 1.0, a22, a22, rest, nn, kk);
+            else
+                wg_gemm<4, kSyrkAdd>(-1.0, l21, l21.tr(), 1.0, a22, a22, rest, nn, kk);
         }
     }
     __syncthreads();
@@ -221,9 +316,8 @@ __device__ __attribute__((noinline)) void wg_trsm_upper(DV U, DV Bm, int n, int 
 // step k does (column c <= nu: S = R + B^T P B is not positive definite).
 constexpr int kRiccatiStepInfo = 65536;
 
-template <int TN>
-__device__ void riccati_instance(RiccatiInstance const& ins, int horizon, const double* in, double* out, double* work,
-                                 double* sB, double* sW, int* sBad, int32_t* info_out) {
+__device__ void riccati_instance(RiccatiInstance const& ins, int horizon, int tn, const double* in, double* out,
+                                 double* work, double* sB, double* sW, int* sBad, double* sG, int32_t* info_out) {
     int const nx = ins.nx, nu = ins.nu, nt = nx + nu;
     DV const A{const_cast<double*>(in) + ins.a, nx, false};
     DV const B{const_cast<double*>(in) + ins.b, nx, false};
@@ -244,15 +338,15 @@ __device__ void riccati_instance(RiccatiInstance const& ins, int horizon, const 
         Lcur(i, j) = i >= j ? Q(i, j) : 0.0;
     }
     __syncthreads();
-    int b0 = wg_potrf<TN>(Lcur, nx, sB, sW, sBad);
+    int b0 = wg_potrf(Lcur, nx, tn, sB, sW, sBad, sG);
     if (b0) bad = b0;
     for (int k = horizon - 1; k >= 0; --k) {
         // M = L^T [B A]
-        wg_gemm<TN, kUpperA>(1.0, Lcur.tr(), BA, 0.0, Mm, Mm, nx, nt, nx);
+        wg_gemm_lds<kUpperA>(1.0, Lcur.tr(), BA, 0.0, Mm, Mm, nx, nt, nx, tn, sG);
         // Lam = blkdiag(R, Q) (lower) + M^T M, the block diagonal read in
         // the syrk epilogue
-        wg_gemm<TN, kSyrkAdd>(1.0, Mm.tr(), Mm, 1.0, BlkDiagRQ{R, Q, nu}, Lam, nt, nt, nx);
-        int const bk = wg_potrf<TN>(Lam, nt, sB, sW, sBad);
+        wg_gemm_lds<kSyrkAdd>(1.0, Mm.tr(), Mm, 1.0, BlkDiagRQ{R, Q, nu}, Lam, nt, nt, nx, tn, sG);
+        int const bk = wg_potrf(Lam, nt, tn, sB, sW, sBad, sG);
         if (bk && bad == 0) bad = kRiccatiStepInfo * (k + 1) + bk;  // step k, column bk of Lam
         // K_k = -Ls^{-T} Lks^T
         DV const Kk{out + ins.k + int64_t(k) * nu * nx, nu, false};
@@ -278,6 +372,7 @@ __global__ __launch_bounds__(kT, 2) void riccati_persistent_kernel(const Riccati
                                                                 RiccatiTuning tuning, int32_t* info) {
     __shared__ double sB[kTileElems];
     __shared__ double sW[kTileElems];
+    __shared__ double sG[kGemmLds];
     __shared__ int sIdx, sBad;
     for (;;) {
         if (threadIdx.x == 0) sIdx = atomicAdd(&queue[0], 1);
@@ -289,12 +384,7 @@ __global__ __launch_bounds__(kT, 2) void riccati_persistent_kernel(const Riccati
         RiccatiInstance const ins = inst[idx];
         int const tn = tuning.tn[min(ins.nx / 16, 15)];
         int32_t* inf = info ? info + idx : nullptr;
-        if (tn == 1)
-            riccati_instance<1>(ins, horizon, in, out, work, sB, sW, &sBad, inf);
-        else if (tn == 2)
-            riccati_instance<2>(ins, horizon, in, out, work, sB, sW, &sBad, inf);
-        else
-            riccati_instance<4>(ins, horizon, in, out, work, sB, sW, &sBad, inf);
+        riccati_instance(ins, horizon, tn, in, out, work, sB, sW, &sBad, sG, inf);
     }
 }
 
