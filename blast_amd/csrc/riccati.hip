@@ -123,7 +123,9 @@ __device__ void wg_gemm_lds(double alpha, DV A, DV B, double beta, CV C, DV D, i
     using Mf = Mfma<double>;
     constexpr int BM = 64, KC = kGemmKc, LS = kGemmLs, TN = 4;
     int const lbn = tn == 4 ? 6 : (tn == 2 ? 5 : 4), BN = 1 << lbn;
-    int const tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // (lane-dependent indices formed here: hoisted out of a caller's loop they
+    // would stay live, and be saved, across its calls)
+    int const tid = opaque_v(int(threadIdx.x)), lane = tid & 63, w = tid >> 6;
     int const m = lane & 15, g = lane >> 4;
     int const MB = (M + BM - 1) / BM, NB = (N + BN - 1) >> lbn;
     double av[4], bv[TN];
@@ -257,12 +259,7 @@ __device__ int wg_potrf(DV a, int n, int tn, double* sB, double* sW, int* sBad, 
             int const gi = (k0 >> 4) & 3;
             DV const l21 = a.sub(r0, k0 - 16 * gi), a22 = a.sub(r0, r0);
             int const nn = gi == 3 ? rest : min(16, rest), kk = 16 * gi + kb;
-            if (tn == 1)
-                wg_gemm<1, kSyrkAdd>(-1.0, l21, l21.tr(), 1.0, a22, a22, rest, nn, kk);
-            else if (tn == 2)
-                wg_gemm<2, kSyrkAdd>(-1.0, l21, l21.tr(), 1.0, a22, a22, rest, nn, kk);
-            else
-                wg_gemm<4, kSyrkAdd>(-1.0, l21, l21.tr(), 1.0, a22, a22, rest, nn, kk);
+            wg_gemm_lds<kSyrkAdd>(-1.0, l21, l21.tr(), 1.0, a22, a22, rest, nn, kk, tn, sG);
         }
     }
     __syncthreads();
