@@ -547,6 +547,47 @@ def adjoint_workload(steps, warmup, batch=4096, nmin=32, nmax=192, seed=20240611
                 worst = max(worst, np.abs(o["Ab"].logical(ent["count"])[0] - ab_ref).max() / np.abs(ab_ref).max(),
                             np.abs(o["Bb"].logical(ent["count"])[0] - bb_ref).max() / np.abs(bb_ref).max())
             res[kind]["max_rel_err_sample_vs_fp64_reference"] = float(worst)
+        if kind == "fp64_backward":
+            # the same adjoints as ONE persistent launch over all sizes
+            # (blast_hip_dposv_backward_uneven_batched, from L and x of the
+            # forward), into separate outputs, checked against the per-size ones
+            ainst = (capi.AdjointInstance * batch)()
+            nlist, k, outs = [], 0, []
+            for ent in groups:
+                o = ent["float64"]
+                la, lb = o["A"].layout, o["B"].layout
+                ab2, bb2 = torch.zeros_like(o["Ab"].tensor), torch.zeros_like(o["Bb"].tensor)
+                outs.append((o, ab2, bb2))
+                for i in range(ent["count"]):
+                    ainst[k].l = la.descriptor(o["L"].tensor.data_ptr() + 8 * i * la.batch_stride)
+                    ainst[k].x = lb.descriptor(o["X"].tensor.data_ptr() + 8 * i * lb.batch_stride)
+                    ainst[k].xb = lb.descriptor(o["Xb"].tensor.data_ptr() + 8 * i * lb.batch_stride)
+                    ainst[k].ab = la.descriptor(ab2.data_ptr() + 8 * i * la.batch_stride)
+                    ainst[k].bb = lb.descriptor(bb2.data_ptr() + 8 * i * lb.batch_stride)
+                    nlist.append(ent["n"])
+                    k += 1
+            dainst = torch.from_numpy(np.frombuffer(bytes(ainst), dtype=np.uint8).copy()).cuda()
+            aorder = np.zeros(batch, dtype=np.int32)
+            capi.check("blast_hip_uneven_order", capi.lib().blast_hip_uneven_order(
+                batch, np.array(nlist, dtype=np.int32).ctypes.data, aorder.ctypes.data))
+            aqueue = torch.from_numpy(np.concatenate([[0], aorder]).astype(np.int32)).cuda()
+            ainfo = torch.zeros(batch, dtype=torch.int32, device="cuda")
+
+            def one_backward():
+                capi.check("blast_hip_dposv_backward_uneven_batched",
+                           capi.lib().blast_hip_dposv_backward_uneven_batched(
+                               torch.cuda.current_stream().cuda_stream, dainst.data_ptr(), batch, aqueue.data_ptr(),
+                               ainfo.data_ptr()))
+
+            t_one = time_steps(one_backward, steps, warmup)
+            assert int(ainfo.abs().sum()) == 0
+            diff = max(max(float((ab2 - o["Ab"].tensor).abs().max() / o["Ab"].tensor.abs().max()),
+                           float((bb2 - o["Bb"].tensor).abs().max() / o["Bb"].tensor.abs().max()))
+                       for o, ab2, bb2 in outs)
+            res["fp64_backward_uneven_one_launch"] = {
+                "instances_per_s": batch / t_one, "ms_per_step": t_one * 1e3,
+                "max_rel_diff_vs_per_size_backward": diff,
+                "kernel": "posv_backward_uneven_kernel (one persistent launch, queue ordered by block count)"}
     return res
 
 

@@ -83,6 +83,7 @@ def lib():
                                                i32p]
         L.blast_hip_uneven_order.argtypes = [i64, vp, vp]
         L.blast_hip_dposv_uneven_batched.argtypes = [vp, vp, i64, vp, vp]
+        L.blast_hip_dposv_backward_uneven_batched.argtypes = [vp, vp, i64, vp, vp]
         L.blast_hip_dtrsv_batched.argtypes = [vp, c_int, P, P, P, i64]
         L.blast_hip_strsv_batched.argtypes = [vp, c_int, P, P, P, i64]
         L.blast_hip_dsgesv_noalloc_batched.argtypes = [vp, c_int, c_int, ctypes.c_double, P, vp, i64, P, P, i64,
@@ -209,6 +210,12 @@ class SolveInstance(ctypes.Structure):
     _fields_ = [("a", BlastMatrix), ("l", BlastMatrix), ("b", BlastMatrix), ("x", BlastMatrix)]
 
 
+class AdjointInstance(ctypes.Structure):
+    """blast_adjoint_instance_t: descriptors of one instance of an uneven adjoint."""
+    _fields_ = [("l", BlastMatrix), ("x", BlastMatrix), ("xb", BlastMatrix), ("ab", BlastMatrix),
+                ("bb", BlastMatrix)]
+
+
 class UnevenBatch:
     """Instances of different n packed back to back in one FP64 buffer per
     operand (A, L, B, X), each in its own Layout; the device descriptor array
@@ -268,8 +275,44 @@ class UnevenBatch:
                      lib().blast_hip_dposv_uneven_batched(_stream(stream), self.inst.data_ptr(), self.count,
                                                           self.queue.data_ptr(), self.info.data_ptr()))
 
+    def backward_buffers(self, device="cuda"):
+        """Xbar, Abar and Bbar buffers (laid out as B, A and B) and the adjoint
+        descriptor array over this batch's L and X (the forward's outputs)."""
+        import torch
+
+        assert self.lb is not None, "the adjoint needs the right-hand sides' layouts"
+        self.xb = torch.zeros_like(self.b)
+        self.bb = torch.zeros_like(self.b)
+        self.ab = torch.zeros_like(self.a)
+        inst = (AdjointInstance * max(self.count, 1))()
+        for k in range(self.count):
+            inst[k].l = self.la[k].descriptor(self.l.data_ptr() + 8 * self.oa[k])
+            inst[k].x = self.lb[k].descriptor(self.x.data_ptr() + 8 * self.ob[k])
+            inst[k].xb = self.lb[k].descriptor(self.xb.data_ptr() + 8 * self.ob[k])
+            inst[k].ab = self.la[k].descriptor(self.ab.data_ptr() + 8 * self.oa[k])
+            inst[k].bb = self.lb[k].descriptor(self.bb.data_ptr() + 8 * self.ob[k])
+        raw = np.frombuffer(bytes(inst), dtype=np.uint8)
+        self.adj_inst = torch.from_numpy(raw.copy()).to(device)
+        self.adj_queue = self.queue.clone()
+        self.adj_info = torch.full((max(self.count, 1),), -2, dtype=torch.int32, device=device)
+
+    def pack_xbar(self, xb_list):
+        hb = np.zeros(self.xb.numel())
+        for k, lay in enumerate(self.lb):
+            hb[self.ob[k]:self.ob[k] + lay.batch_stride] = lay.pack(xb_list[k])
+        self.xb.copy_(__import__("torch").from_numpy(hb))
+
+    def backward(self, stream=None):
+        return check("blast_hip_dposv_backward_uneven_batched",
+                     lib().blast_hip_dposv_backward_uneven_batched(_stream(stream), self.adj_inst.data_ptr(),
+                                                                   self.count, self.adj_queue.data_ptr(),
+                                                                   self.adj_info.data_ptr()))
+
     def result(self, which, k):
-        buf, lays, offs = ((self.l, self.la, self.oa) if which == "l" else (self.x, self.lb, self.ob))
+        bufs = {"l": (self.l, self.la, self.oa), "x": (getattr(self, "x", None), self.lb, getattr(self, "ob", None)),
+                "ab": (getattr(self, "ab", None), self.la, self.oa),
+                "bb": (getattr(self, "bb", None), self.lb, getattr(self, "ob", None))}
+        buf, lays, offs = bufs[which]
         lay = lays[k]
         host = buf[offs[k]:offs[k] + lay.batch_stride].cpu().numpy()
         return lay.unpack(host, 1)[0]

@@ -541,6 +541,15 @@ int blast_hip_dposv_uneven_batched(void* s, const blast_solve_instance_t* inst, 
     return launch_posv_uneven(hipStream_t(s), inst, count, queue, info, 0);
 }
 
+int blast_hip_dposv_backward_uneven_batched(void* s, const blast_adjoint_instance_t* inst, int64_t count,
+                                            int32_t* queue, int32_t* info) {
+    if (count < 0 || count > 0x7fffffff - 4096) { set_error("posv_backward_uneven: count out of range"); return -3; }
+    if (count > 0 && !inst) { set_error("posv_backward_uneven: null instance array"); return -2; }
+    if (count > 0 && !queue) { set_error("posv_backward_uneven: null queue"); return -4; }
+    (void)hipGetLastError();
+    return launch_posv_backward_uneven(hipStream_t(s), inst, count, queue, info, 0);
+}
+
 int blast_hip_dsgesv_noalloc_batched(void* s, int precision, int itmax, double tol, const blast_matrix_t* A,
                                      int64_t* ipiv, int64_t ipiv_stride, const blast_matrix_t* B,
                                      const blast_matrix_t* X, int64_t batch, int32_t* info, int32_t* iter) {

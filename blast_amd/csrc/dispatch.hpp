@@ -71,6 +71,9 @@ int launch_chol_team(hipStream_t, Mat<double> A, Mat<double> L, Mat<double> B, M
 int launch_posv_uneven(hipStream_t, const blast_solve_instance_t* inst, int64_t count, int32_t* queue,
                        int32_t* info, int grid);
 int uneven_max_n();
+// adjoint_uneven.hip: the posv adjoint over an uneven batch, one launch.
+int launch_posv_backward_uneven(hipStream_t, const blast_adjoint_instance_t* inst, int64_t count, int32_t* queue,
+                                int32_t* info, int grid);
 // sposv_ir.hip: returns 1 when the shape is not covered
 int launch_sposv_ir(hipStream_t, int bf16, Mat<double> A, Mat<double> B, Mat<double> X, int itmax, double tol,
                     int64_t batch, int32_t* info, int32_t* iter);

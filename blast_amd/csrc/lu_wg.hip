@@ -216,9 +216,6 @@ __global__ __launch_bounds__(kLuThreads, 2) void lu_wg_kernel(LuArgs<T> g) {
         double const v = double(sa.at(sa.rows(min(i, n - 1)) + sa.cols(min(j, n - 1))));
         return (i < n && j < n) ? v : (i == j ? 1.0 : 0.0);
     };
-    auto fset = [&](int i, int j, double v) {
-        if (i < n && j < n) sa.at(sa.rows(i) + sa.cols(j)) = T(v);
-    };
     // the right-hand sides ride along as column tile NT of [A | X]: the row
     // swaps and the elimination turn X = B into y = L^{-1} P B
     Sep<T> const sx = Sep<T>::make(g.X, rhs ? g.X.inst(b) : nullptr);

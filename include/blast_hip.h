@@ -325,6 +325,23 @@ int blast_hip_uneven_order(int64_t count, const int32_t* n, int32_t* order);
 int blast_hip_dposv_uneven_batched(void* stream, const blast_solve_instance_t* inst, int64_t count, int32_t* queue,
                                    int32_t* info);
 
+/* Uneven batched adjoint of the SPD solve (FP64), from the factor kept by
+ * the forward pass: per instance Bbar = A^{-1} Xbar = L^{-T} L^{-1} Xbar and
+ * Abar = -fold(Bbar X^T) (lower entries, 0 above), as
+ * blast_hip_dposv_backward_batched with recompute = 0, for instances of
+ * different n (<= blast_hip_uneven_max_n()) and 1..8 columns of X in ONE
+ * persistent launch.  No counterpart in the reference (BLAST has no
+ * adjoints); it serves callers that differentiate through uneven batches of
+ * blast_hip_dposv_uneven_batched.  Descriptors per instance: l (n x n
+ * lower), x and xb (n x r), ab (n x n, written whole), bb (n x r); any
+ * storage.  queue and info as for blast_hip_dposv_uneven_batched (info 0,
+ * or -1 for an instance whose descriptors are inconsistent). */
+typedef struct blast_adjoint_instance_t {
+    blast_matrix_t l, x, xb, ab, bb;
+} blast_adjoint_instance_t;
+int blast_hip_dposv_backward_uneven_batched(void* stream, const blast_adjoint_instance_t* inst, int64_t count,
+                                            int32_t* queue, int32_t* info);
+
 /* ---- Uneven batched Riccati recursion (LQR / MPC backward pass), FP64 ----
  * No BLAST function is replaced: this composite strings together the entry
  * points above (gemm -> syrkLower -> potrf -> trsm per step) the way the
