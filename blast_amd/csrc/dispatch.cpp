@@ -19,6 +19,92 @@ int launch_chol_fused(hipStream_t s, Mat<double> A, Mat<double> L, Mat<double> B
     return launch_chol_owner(s, A, L, B, X, batch, info, only_if);
 }
 
+// ---- Blocked Cholesky path for n > 256 (the MFMA kernels above cover
+// n <= 256): 256-row blocks, each the fused n <= 256 kernels, trsm_tiles and
+// the syrk / gemm tile kernels on sub-blocks.  Stream-ordered launches, no
+// allocation.  (BLAST's own blocked algorithms: dense/Potrf.hpp:66-106 and
+// dense/Trsm.hpp:17-66.)
+constexpr int kBlockedNb = 256;
+
+// Rows i0.., columns j0.. (logical) of every instance: the same storage
+// with the base moved; in panel storage the first storage row must lie on a
+// panel boundary (i0, j0 are multiples of 256 here).
+template <typename T>
+Mat<T> subview(Mat<T> m, int i0, int j0, int rows, int cols) {
+    constexpr int PS = PanelSize<T>::value;
+    int const si = m.trans ? j0 : i0, sj = m.trans ? i0 : j0;
+    int64_t const off = m.panel ? int64_t(si / PS) * m.spacing + si % PS + int64_t(sj) * PS
+                                : int64_t(si) + int64_t(m.spacing) * sj;
+    Mat<T> r = m;
+    r.data = m.data + off;
+    r.rows = rows;
+    r.cols = cols;
+    return r;
+}
+
+// X = A^{-1} B for A lower (upper = 0) or upper triangular, n > 256, by
+// blocks: the diagonal solve on trsm_tiles, the rest of the right-hand sides
+// updated by gemm.  X may alias B.
+template <typename T>
+int trsm_left_blocked(hipStream_t s, int upper, int unit, T alpha, Mat<T> A, Mat<T> B, Mat<T> X, int64_t batch) {
+    int const n = A.rows, nrhs = X.cols, NB = kBlockedNb;
+    if (X.data != B.data || alpha != T(1)) {
+        int const r = launch_copy_scaled<T>(s, B, X, alpha, batch);
+        if (r) return r;
+    }
+    int const nblk = (n + NB - 1) / NB;
+    for (int t = 0; t < nblk; ++t) {
+        int const kb = upper ? nblk - 1 - t : t, k0 = kb * NB, nb = n - k0 < NB ? n - k0 : NB;
+        Mat<T> const Akk = subview(A, k0, k0, nb, nb), Xk = subview(X, k0, 0, nb, nrhs);
+        int r = launch_trsm_tiles<T>(s, upper, unit, T(1), Akk, Xk, Xk, batch);
+        if (r) return r;
+        if (!upper && k0 + nb < n) {  // rows below: X2 -= A21 X1
+            int const rest = n - k0 - nb;
+            Mat<T> const X2 = subview(X, k0 + nb, 0, rest, nrhs);
+            r = launch_gemm_tiles<T>(s, T(-1), subview(A, k0 + nb, k0, rest, nb), Xk, T(1), X2, X2, batch);
+        } else if (upper && k0 > 0) {  // rows above: X0 -= A01 X1
+            Mat<T> const X0 = subview(X, 0, 0, k0, nrhs);
+            r = launch_gemm_tiles<T>(s, T(-1), subview(A, 0, k0, k0, nb), Xk, T(1), X0, X0, batch);
+        }
+        if (r) return r;
+    }
+    return 0;
+}
+
+// L = chol(A), n > 256, right-looking by 256-column blocks: the diagonal
+// block on the fused n <= 256 kernel, the panel by a right-side trsm
+// (L21 L11^T = A21), the trailing matrix by syrk.  The lower triangle of A
+// is copied into L first (unless L is A); the rest of L is left as the
+// kernels leave it for n <= 256 (zeros in the 4x4 diagonal panel blocks).
+// info: from the factor's diagonal (a pivot that is not positive leaves a
+// NaN or non-positive entry there and NaN in the later ones).
+template <typename T>
+int potrf_blocked(hipStream_t s, Mat<T> A, Mat<T> L, int64_t batch, int32_t* info) {
+    int const n = A.rows, NB = kBlockedNb;
+    int r = 0;
+    if (A.data != L.data) r = launch_copy_lower<T>(s, A, L, batch);
+    if (r) return r;
+    Mat<T> const none{};
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        int const nb = n - k0 < NB ? n - k0 : NB, rest = n - k0 - nb;
+        Mat<T> const L11 = subview(L, k0, k0, nb, nb);
+        if constexpr (std::is_same_v<T, double>)
+            r = launch_chol_fused(s, L11, L11, none, none, batch, nullptr);
+        else
+            r = launch_chol_owner_f32(s, L11, L11, none, none, batch, nullptr);
+        if (r) return r < 0 ? r : BLAST_HIP_ERR_RUNTIME;
+        if (rest > 0) {
+            Mat<T> const L21 = subview(L, k0 + nb, k0, rest, nb), L22 = subview(L, k0 + nb, k0 + nb, rest, rest);
+            // X L11^T = L21  <=>  L11 X^T = L21^T
+            r = launch_trsm_tiles<T>(s, 0, 0, T(1), L11, transposed(L21), transposed(L21), batch);
+            if (r) return r;
+            r = launch_syrk_tiles<T>(s, T(-1), L21, T(1), L22, L22, batch);
+            if (r) return r;
+        }
+    }
+    return launch_diag_info<T>(s, L, batch, info);
+}
+
 template <typename T>
 int dispatch_gemm(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D, int64_t batch) {
     if (algorithm() == BLAST_HIP_ALGO_AUTO) return launch_gemm_tiles<T>(s, alpha, A, B, beta, C, D, batch);
@@ -46,6 +132,10 @@ int dispatch_trsm(hipStream_t s, int side_right, int upper, int unit, T alpha, M
                                                         transposed(X), batch)
                                  : launch_trsm_tiles<T>(s, upper, unit, alpha, A, B, X, batch);
         if (r != 1) return r;
+        if (A.rows == A.cols && A.rows > 16 * 16)  // n > 256: by 256-row blocks
+            return side_right ? trsm_left_blocked<T>(s, !upper, unit, alpha, transposed(A), transposed(B),
+                                                     transposed(X), batch)
+                              : trsm_left_blocked<T>(s, upper, unit, alpha, A, B, X, batch);
     }
     return launch_trsm_ref<T>(s, side_right, upper, unit, alpha, A, B, X, batch);
 }
@@ -60,6 +150,7 @@ int dispatch_potrf(hipStream_t s, Mat<T> A, Mat<T> L, int64_t batch, int32_t* in
         else
             r = launch_chol_owner_f32(s, A, L, none, none, batch, info);
         if (r != 1) return r;
+        if (A.rows == A.cols && A.rows > kBlockedNb) return potrf_blocked<T>(s, A, L, batch, info);
     }
     return launch_potrf_ref<T>(s, A, L, batch, info);
 }
@@ -87,6 +178,13 @@ int dispatch_posv(hipStream_t s, Mat<T> A, Mat<T> L, Mat<T> B, Mat<T> X, int64_t
         else
             r = launch_chol_owner_f32(s, A, L, B, X, batch, info);
         if (r != 1) return r;
+        if (A.rows == A.cols && A.rows > kBlockedNb) {
+            r = potrf_blocked<T>(s, A, L, batch, info);
+            if (r) return r;
+            r = trsm_left_blocked<T>(s, 0, 0, T(1), L, B, X, batch);
+            if (r) return r;
+            return trsm_left_blocked<T>(s, 1, 0, T(1), transposed(L), X, X, batch);
+        }
     }
     int r = launch_potrf_ref<T>(s, A, L, batch, info);
     if (r) return r;

@@ -112,6 +112,11 @@ struct RiccatiTuning {
 int launch_riccati(hipStream_t, int horizon, const void* inst, int64_t start, int64_t count, const double* in,
                    double* out, double* work, int32_t* queue, RiccatiTuning const& tuning, int32_t* info, int grid);
 
+// blocked.hip: helpers of the blocked Cholesky path for n > 256
+template <typename T> int launch_copy_lower(hipStream_t, Mat<T> A, Mat<T> L, int64_t batch);
+template <typename T> int launch_copy_scaled(hipStream_t, Mat<T> B, Mat<T> X, T alpha, int64_t batch);
+template <typename T> int launch_diag_info(hipStream_t, Mat<T> L, int64_t batch, int32_t* info);
+
 // dispatch.cpp
 template <typename T> int dispatch_gemm(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t);
 template <typename T> int dispatch_syrk(hipStream_t, T, Mat<T>, T, Mat<T>, Mat<T>, int64_t);
