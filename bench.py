@@ -53,7 +53,7 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--riccati-count", type=int, default=20000, help="instances of the uneven Riccati workload")
-    p.add_argument("--only", choices=("all", "posv", "gemm", "riccati", "mixed256", "adjoint", "convert"),
+    p.add_argument("--only", choices=("all", "posv", "gemm", "riccati", "mixed256", "adjoint", "convert", "blocked"),
                    default="all",
                    help="run one workload alone (diagnostics; the bench line needs 'all')")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -164,6 +164,55 @@ def gemm_workload(steps, warmup, batch=4096, n=64):
     return {"gflops": flops / t / 1e9, "instances_per_s": batch / t, "ms_per_step": t * 1e3,
             "hbm_gbs_algorithmic": hbm / t / 1e9, "hbm_frac": hbm / t / 1e9 / PEAK_HBM_GBS,
             "kernel": "gemm_small_kernel<double>"}
+
+
+def blocked_workload(steps, warmup, batch=256, n=512, nrhs=4):
+    """dposv beyond the n <= 256 kernels: n = 512, 4 RHS, batch 256, panel
+    storage, A = G G^T + n I: the AUTO blocked path (256-row blocks on the
+    MFMA kernels) against the reference-order column sweeps it replaces
+    (timed once: it is slow), with the backward error of both."""
+    import torch
+
+    from blast_amd import capi
+    from blast_amd.layout import Layout
+
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    g = torch.randn(batch, n, n, dtype=torch.float64, device="cuda", generator=gen)
+    a = torch.bmm(g, g.transpose(1, 2).contiguous()) + n * torch.eye(n, dtype=torch.float64, device="cuda")
+    b = torch.rand(batch, n, nrhs, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
+    la, lb = Layout.make(n, n), Layout.make(n, nrhs)
+
+    def pack(x, lay):
+        idx = torch.from_numpy(lay.index_map().reshape(-1)).cuda()
+        buf = torch.zeros(batch * lay.batch_stride, dtype=torch.float64, device="cuda")
+        buf.view(batch, lay.batch_stride).index_copy_(1, idx, x.reshape(batch, -1))
+        return capi.Operand(lay, buf)
+
+    A, B = pack(a, la), pack(b, lb)
+    L, X = capi.Operand(la, torch.zeros_like(A.tensor)), capi.Operand(lb, torch.zeros_like(B.tensor))
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    ib = torch.from_numpy(lb.index_map().reshape(-1)).cuda()
+
+    def bwd():
+        x = X.tensor.view(batch, lb.batch_stride)[:, ib].reshape(batch, n, nrhs)
+        r = b - torch.bmm(a, x)
+        return float((torch.linalg.matrix_norm(r) / (torch.linalg.matrix_norm(a) * torch.linalg.matrix_norm(x)
+                                                   + torch.linalg.matrix_norm(b))).max())
+
+    out = {"batch": batch, "n": n, "nrhs": nrhs}
+    capi.set_algorithm(capi.ALGO_AUTO)
+    t = time_steps(lambda: capi.posv(A, L, B, X, batch, info), steps, warmup)
+    flops = batch * (n ** 3 / 3 + 2 * n * n * nrhs)
+    out["auto_blocked"] = {"solves_per_s": batch / t, "ms_per_step": t * 1e3, "fp64_tflops": flops / t / 1e12,
+                           "max_solve_backward_error": bwd(), "info_nonzero": int((info != 0).sum())}
+    capi.set_algorithm(capi.ALGO_REFERENCE)
+    try:
+        t_ref = time_steps(lambda: capi.posv(A, L, B, X, batch, info), 1, 0)
+    finally:
+        capi.set_algorithm(capi.ALGO_AUTO)
+    out["reference_column_sweeps"] = {"solves_per_s": batch / t_ref, "ms_per_step": t_ref * 1e3,
+                                      "max_solve_backward_error": bwd()}
+    return out
 
 
 def convert_workload(steps, warmup, batch=4096, n=128):
@@ -650,12 +699,13 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if args.only in ("gemm", "riccati", "mixed256", "adjoint", "convert"):
+    if args.only in ("gemm", "riccati", "mixed256", "adjoint", "convert", "blocked"):
         res = {"gemm": lambda: gemm_workload(args.steps, args.warmup),
                "riccati": lambda: riccati_workload(args.steps, args.warmup, args.riccati_count),
                "mixed256": lambda: mixed256_workload(args.steps, args.warmup),
                "adjoint": lambda: adjoint_workload(args.steps, args.warmup),
-               "convert": lambda: convert_workload(args.steps, args.warmup)}[args.only]()
+               "convert": lambda: convert_workload(args.steps, args.warmup),
+               "blocked": lambda: blocked_workload(args.steps, args.warmup)}[args.only]()
         print(json.dumps({args.only: res}))
         return
     batch = args.batch
@@ -775,6 +825,7 @@ def main():
         f"riccati_uneven_{args.riccati_count}_h10": riccati_workload(args.steps, args.warmup, args.riccati_count),
         "mixed_n256_nrhs8_batch2048": mixed256_workload(min(args.steps, 5), min(args.warmup, 1)),
         "adjoint_spd_solve_n32_192_batch4096": adjoint_workload(min(args.steps, 5), min(args.warmup, 1)),
+        "posv_blocked_n512_nrhs4_batch256": blocked_workload(min(args.steps, 5), min(args.warmup, 1)),
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(la, lb, abuf, bbuf, args.cpu_seconds)
