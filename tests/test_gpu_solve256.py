@@ -208,6 +208,12 @@ def test_mixed_256_conditioned(precision, spd_variant):
     if not spd_variant and np.any(conv):
         piv = ipiv.cpu().numpy().reshape(batch, n)
         assert np.all((piv >= np.arange(n)) & (piv < n))
+    # agreement with the FP64 solution where the conditioning allows it: the
+    # forward error is at most about kappa times the backward error
+    well = kap <= 1e4
+    x_ref = np.linalg.solve(a[well], b[well])
+    ferr = np.abs(x[well] - x_ref).max(axis=(1, 2)) / np.abs(x_ref).max(axis=(1, 2))
+    assert np.all(ferr <= kap[well] * 1e-12), ferr
 
 
 def test_mixed_small_shapes_and_fallback_flags():
