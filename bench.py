@@ -167,10 +167,11 @@ def gemm_workload(steps, warmup, batch=4096, n=64):
 
 
 def blocked_workload(steps, warmup, batch=256, n=512, nrhs=4):
-    """dposv beyond the n <= 256 kernels: n = 512, 4 RHS, batch 256, panel
-    storage, A = G G^T + n I: the AUTO blocked path (256-row blocks on the
-    MFMA kernels) against the reference-order column sweeps it replaces
-    (timed once: it is slow), with the backward error of both."""
+    """dposv and dgesv beyond the n <= 256 kernels: n = 512, 4 RHS, batch
+    256.  dposv: panel storage, A = G G^T + n I, the AUTO blocked path
+    (256-row blocks on the MFMA kernels); dgesv: the blocked LU.  Each
+    against the reference-order column sweeps it replaces (timed once: they
+    are slow), with the backward error of both."""
     import torch
 
     from blast_amd import capi
@@ -212,6 +213,44 @@ def blocked_workload(steps, warmup, batch=256, n=512, nrhs=4):
         capi.set_algorithm(capi.ALGO_AUTO)
     out["reference_column_sweeps"] = {"solves_per_s": batch / t_ref, "ms_per_step": t_ref * 1e3,
                                       "max_solve_backward_error": bwd()}
+
+    # dgesv (LU with partial pivoting, in place of A), dense column-major,
+    # A ~ U(-1, 1) + I/2: the AUTO blocked LU (32-column register panels +
+    # gemm tile updates) + getrs against the reference column sweeps.  Each
+    # step first restores A from a pristine copy (timed alone as copy_ms).
+    lc, lbc = Layout.make(n, n, panel=False), Layout.make(n, nrhs, panel=False)
+    ag = torch.rand(batch, n, n, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
+    ag += 0.5 * torch.eye(n, dtype=torch.float64, device="cuda")
+    A0, Bg = pack(ag, lc), pack(b, lbc)
+    Aw, Xg = capi.Operand(lc, torch.empty_like(A0.tensor)), capi.Operand(lbc, torch.zeros_like(Bg.tensor))
+    ipiv = torch.zeros(batch * n, dtype=torch.int64, device="cuda")
+    ibc = torch.from_numpy(lbc.index_map().reshape(-1)).cuda()
+
+    def gesv_step():
+        Aw.tensor.copy_(A0.tensor)
+        capi.gesv(Aw, ipiv, n, Bg, Xg, batch, info)
+
+    def bwd_g():
+        x = Xg.tensor.view(batch, lbc.batch_stride)[:, ibc].reshape(batch, n, nrhs)
+        r = b - torch.bmm(ag, x)
+        return float((torch.linalg.matrix_norm(r) / (torch.linalg.matrix_norm(ag) * torch.linalg.matrix_norm(x)
+                                                   + torch.linalg.matrix_norm(b))).max())
+
+    t_copy = time_steps(lambda: Aw.tensor.copy_(A0.tensor), steps, warmup)
+    t = time_steps(gesv_step, steps, warmup)
+    gflops = batch * (2 * n ** 3 / 3 + 2 * n * n * nrhs)
+    g = {"layout": "dense column-major", "copy_ms": t_copy * 1e3, "ms_per_step_incl_copy": t * 1e3,
+         "ms_per_step": (t - t_copy) * 1e3, "solves_per_s": batch / (t - t_copy),
+         "fp64_tflops": gflops / (t - t_copy) / 1e12, "max_solve_backward_error": bwd_g(),
+         "info_nonzero": int((info != 0).sum())}
+    capi.set_algorithm(capi.ALGO_REFERENCE)
+    try:
+        t_ref = time_steps(gesv_step, 1, 0)
+    finally:
+        capi.set_algorithm(capi.ALGO_AUTO)
+    g["reference_column_sweeps"] = {"ms_per_step_incl_copy": t_ref * 1e3, "solves_per_s": batch / (t_ref - t_copy),
+                                    "max_solve_backward_error": bwd_g()}
+    out["dgesv"] = g
     return out
 
 
@@ -825,7 +864,7 @@ def main():
         f"riccati_uneven_{args.riccati_count}_h10": riccati_workload(args.steps, args.warmup, args.riccati_count),
         "mixed_n256_nrhs8_batch2048": mixed256_workload(min(args.steps, 5), min(args.warmup, 1)),
         "adjoint_spd_solve_n32_192_batch4096": adjoint_workload(min(args.steps, 5), min(args.warmup, 1)),
-        "posv_blocked_n512_nrhs4_batch256": blocked_workload(min(args.steps, 5), min(args.warmup, 1)),
+        "blocked_n512_nrhs4_batch256": blocked_workload(min(args.steps, 5), min(args.warmup, 1)),
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(la, lb, abuf, bbuf, args.cpu_seconds)

@@ -105,6 +105,28 @@ int potrf_blocked(hipStream_t s, Mat<T> A, Mat<T> L, int64_t batch, int32_t* inf
     return launch_diag_info<T>(s, L, batch, info);
 }
 
+// LU with partial pivoting, square 256 < n <= 1024, right-looking by
+// column blocks of the panel kernel's width (24 FP64, 32 FP32): the panel
+// kernel (factor, interchanges on the other columns, U12), then
+// A22 -= L21 U12 on the gemm tile kernel.  info is cleared first; an
+// instance whose panel meets a zero pivot records it and its later panels do
+// nothing (BLAST stops there, dense/Getrf.hpp:52-119).
+template <typename T>
+int getrf_blocked(hipStream_t s, Mat<T> A, int64_t* ipiv, int64_t ipiv_stride, int64_t batch, int32_t* info) {
+    int const n = A.rows, kLuPanel = getrf_panel_width<T>();
+    int r = info ? launch_fill_i32(s, info, 0, batch) : 0;
+    for (int k0 = 0; k0 < n && r == 0; k0 += kLuPanel) {
+        int const nb = n - k0 < kLuPanel ? n - k0 : kLuPanel, rest = n - k0 - nb;
+        r = launch_getrf_panel<T>(s, A, k0, nb, ipiv, ipiv_stride, batch, info);
+        if (r == 0 && rest > 0) {
+            Mat<T> const A22 = subview(A, k0 + nb, k0 + nb, rest, rest);
+            r = launch_gemm_tiles<T>(s, T(-1), subview(A, k0 + nb, k0, rest, nb), subview(A, k0, k0 + nb, nb, rest),
+                                     T(1), A22, A22, batch);
+        }
+    }
+    return r;
+}
+
 template <typename T>
 int dispatch_gemm(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D, int64_t batch) {
     if (algorithm() == BLAST_HIP_ALGO_AUTO) return launch_gemm_tiles<T>(s, alpha, A, B, beta, C, D, batch);
@@ -166,6 +188,8 @@ int dispatch_getrf(hipStream_t s, Mat<T> A, int64_t* ipiv, int64_t ipiv_stride, 
             if (lu_wg_fits_f32(A)) return launch_wg_getrf_f32(s, A, ipiv, ipiv_stride, batch, info);
         }
     }
+    if (algorithm() == BLAST_HIP_ALGO_AUTO && getrf_panel_covers(A.rows, A.cols, A.trans))
+        return getrf_blocked<T>(s, A, ipiv, ipiv_stride, batch, info);
     return launch_getrf_ref<T>(s, A, ipiv, ipiv_stride, batch, info);
 }
 
@@ -226,7 +250,7 @@ int dispatch_dgesv(hipStream_t s, Mat<double> A, int64_t* ipiv, int64_t ipiv_str
                    int64_t batch, int32_t* info) {
     if (algorithm() == BLAST_HIP_ALGO_AUTO && wg_covers(A.rows, X.cols) && lu_wg_fits(A, X, X.cols))
         return launch_wg_gesv(s, A, ipiv, ipiv_stride, B, X, X.cols, batch, info);
-    int const r = launch_getrf_ref<double>(s, A, ipiv, ipiv_stride, batch, info);
+    int const r = dispatch_getrf<double>(s, A, ipiv, ipiv_stride, batch, info);
     if (r) return r;
     return getrs<double>(s, A, ipiv, ipiv_stride, B, X, batch);
 }

@@ -117,6 +117,15 @@ template <typename T> int launch_copy_lower(hipStream_t, Mat<T> A, Mat<T> L, int
 template <typename T> int launch_copy_scaled(hipStream_t, Mat<T> B, Mat<T> X, T alpha, int64_t batch);
 template <typename T> int launch_diag_info(hipStream_t, Mat<T> L, int64_t batch, int32_t* info);
 
+// getrf_blocked.hip: the panel step of the blocked LU (square 256 < n <=
+// 1024, not transposed): factor columns k0..k0+nb-1 (nb <= the panel width) with partial
+// pivoting, apply the interchanges to the other columns and form U12.
+bool getrf_panel_covers(int m, int n, int trans);
+template <typename T> int getrf_panel_width();  // columns per panel (24 FP64, 32 FP32)
+template <typename T>
+int launch_getrf_panel(hipStream_t, Mat<T> A, int k0, int nb, int64_t* ipiv, int64_t ipiv_stride, int64_t batch,
+                       int32_t* info);
+
 // dispatch.cpp
 template <typename T> int dispatch_gemm(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t);
 template <typename T> int dispatch_syrk(hipStream_t, T, Mat<T>, T, Mat<T>, Mat<T>, int64_t);

@@ -1,7 +1,8 @@
 // Batched D = alpha*op(A)*op(B) + beta*C on MFMA tiles (FP64: v_mfma_f64_16x16x4f64,
-// FP32: v_mfma_f32_16x16x4f32), one 256-thread workgroup per instance.
+// FP32: v_mfma_f32_16x16x4f32), 256-thread workgroups: ceil(strips / 4) per
+// instance (grid y), so that each wavefront owns one 16-row strip of D.
 //
-// Each wavefront owns 16-row strips of D (strip R = wavefront, +4, ...) and
+// Each wavefront owns 16-row strips of D (strip R = wavefront + 4 y, ...) and
 // computes up to four 16x16 tiles of a strip at once: per K-chunk of 4 it
 // loads one A-operand element per lane (reused across the four tiles) and one
 // B-operand element per tile.  The epilogue is BLAST's: d = fma(beta, c,
@@ -36,13 +37,28 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(T alpha, Mat<T> A, Mat<T
     int const MT = (Mr + 15) / 16, NTt = (Nc + 15) / 16, KQ = (K + 3) / 4;
     int const m = lane & 15, g = lane >> 4;
 
-    for (int R = w; R < MT; R += 4) {
+    // row strips R = w + 4 blockIdx.y, + 4 gridDim.y, ...: large outputs
+    // spread over several workgroups per instance (occupancy)
+    for (int R = w + 4 * int(blockIdx.y); R < MT; R += 4 * int(gridDim.y)) {
         int const row = 16 * R + m;  // A-operand row of this lane
         for (int T0 = 0; T0 < NTt; T0 += 4) {
             if (MODE == kSyrkLower && 16 * T0 > 16 * R + 15) break;  // strictly upper tiles
             acc_t acc[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) acc[t] = acc_t{};
+            // C tiles (modes that read C) loaded with the first operands, in
+            // flight during the MFMAs instead of after them
+            constexpr bool kReadC = MODE == kGemm || MODE == kSyrkLower;
+            T cv[4][4];
+            if constexpr (kReadC) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        int const i = 16 * R + M::row(lane, r), j = 16 * (T0 + t) + acc_col(lane);
+                        cv[t][r] = (i < Mr && j < Nc && (MODE != kSyrkLower || i >= j)) ? c(i, j) : T(0);
+                    }
+            }
             constexpr int U = 4;  // K-chunks of loads in flight (8 costs occupancy)
             for (int q0 = 0; q0 < KQ; q0 += U) {
                 T av[U], bv[U][4];
@@ -72,10 +88,10 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(T alpha, Mat<T> A, Mat<T
                     int const i = 16 * R + M::row(lane, r), j = 16 * (T0 + t) + acc_col(lane);
                     if (i < Mr && j < Nc && (MODE != kSyrkLower || i >= j)) {
                         T const ab = acc[t][r] * alpha;
-                        d(i, j) = (MODE == kTrmmLeftUpper || MODE == kTrmmRightLower || MODE == kUpperTimesLower ||
-                                   MODE == kGemmOverwrite)
-                                      ? ab
-                                                                                       : fma_t(beta, c(i, j), ab);
+                        if constexpr (kReadC)
+                            d(i, j) = fma_t(beta, cv[t][r], ab);
+                        else
+                            d(i, j) = ab;
                     }
                 }
         }
@@ -156,7 +172,10 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(T alpha, Mat<T> A, Mat<
 template <typename T, int MODE>
 static int launch_mode(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D, int K, int64_t batch) {
     if (batch == 0 || D.rows == 0 || D.cols == 0) return 0;
-    hipLaunchKernelGGL((gemm_tile_kernel<T, MODE>), dim3(unsigned(batch)), dim3(256), 0, s, alpha, A, B, beta, C, D, K);
+    // one row strip per wavefront: ceil(strips / 4) workgroups per instance
+    int const slices = (((D.rows + 15) / 16) + 3) / 4;
+    hipLaunchKernelGGL((gemm_tile_kernel<T, MODE>), dim3(unsigned(batch), unsigned(slices)), dim3(256), 0, s, alpha, A,
+                       B, beta, C, D, K);
     return check_launch("gemm_tile_kernel");
 }
 

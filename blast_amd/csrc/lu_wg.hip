@@ -91,33 +91,6 @@ __device__ __forceinline__ void wave_argmax(double& v, int& i) {
     argmax_swap<false>(v, i);
 }
 
-// Wavefront maximum of an FP64 value without NaNs, in every lane: four DPP
-// steps within each 16-lane row, then the two row swaps.
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v) {
-    unsigned long long const u = __builtin_bit_cast(unsigned long long, v);
-    unsigned const lo = unsigned(__builtin_amdgcn_mov_dpp(int(unsigned(u)), CTRL, 0xF, 0xF, false));
-    unsigned const hi = unsigned(__builtin_amdgcn_mov_dpp(int(unsigned(u >> 32)), CTRL, 0xF, 0xF, false));
-    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
-}
-template <bool ROW16>
-__device__ __forceinline__ double swap_max(double v) {
-    unsigned long long const u = __builtin_bit_cast(unsigned long long, v);
-    unsigned l0, l1, h0, h1;
-    lane_swap<ROW16>(unsigned(u), l0, l1);
-    lane_swap<ROW16>(unsigned(u >> 32), h0, h1);
-    return __builtin_fmax(__builtin_bit_cast(double, (static_cast<unsigned long long>(h0) << 32) | l0),
-                          __builtin_bit_cast(double, (static_cast<unsigned long long>(h1) << 32) | l1));
-}
-__device__ __forceinline__ double wave_max(double v) {
-    v = __builtin_fmax(v, dpp_f64<0xB1>(v));   // quad_perm [1,0,3,2]
-    v = __builtin_fmax(v, dpp_f64<0x4E>(v));   // quad_perm [2,3,0,1]
-    v = __builtin_fmax(v, dpp_f64<0x141>(v));  // row_half_mirror
-    v = __builtin_fmax(v, dpp_f64<0x140>(v));  // row_mirror
-    v = swap_max<true>(v);
-    return swap_max<false>(v);
-}
-
 // T: the storage type (the arithmetic is FP64 for FP32 storage too: values
 // are widened on load and rounded on store).
 template <typename T>

@@ -28,6 +28,7 @@ RULES = {
     "lu_wg_kernel": dict(scratch=0, vgpr_spill=0, calls=False),
     "posv_uneven_kernel": dict(scratch=0, vgpr_spill=0, calls=False),
     "posv_backward_uneven_kernel": dict(scratch=0, vgpr_spill=0, calls=False),
+    "getrf_panel_kernel": dict(scratch=0, vgpr_spill=0, sgpr_spill=0, calls=False),
     "sposv_ir_kernel": dict(scratch=0, vgpr_spill=0, calls=True),
     "riccati_persistent_kernel": dict(scratch=12, vgpr_spill=0, calls=True),
 }
