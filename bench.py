@@ -53,7 +53,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--riccati-count", type=int, default=20000, help="instances of the uneven Riccati workload")
-    p.add_argument("--only", choices=("all", "posv", "gemm", "riccati", "mixed256", "adjoint", "convert", "blocked"),
+    p.add_argument("--only", choices=("all", "posv", "gemm", "riccati", "mixed256", "adjoint", "convert", "blocked",
+                                      "blocked_lu"),
                    default="all",
                    help="run one workload alone (diagnostics; the bench line needs 'all')")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -214,10 +215,32 @@ def blocked_workload(steps, warmup, batch=256, n=512, nrhs=4):
     out["reference_column_sweeps"] = {"solves_per_s": batch / t_ref, "ms_per_step": t_ref * 1e3,
                                       "max_solve_backward_error": bwd()}
 
-    # dgesv (LU with partial pivoting, in place of A), dense column-major,
-    # A ~ U(-1, 1) + I/2: the AUTO blocked LU (32-column register panels +
-    # gemm tile updates) + getrs against the reference column sweeps.  Each
-    # step first restores A from a pristine copy (timed alone as copy_ms).
+    out["dgesv"] = blocked_lu_workload(steps, warmup, batch, n, nrhs)
+    return out
+
+
+def blocked_lu_workload(steps, warmup, batch=256, n=512, nrhs=4):
+    """dgesv (LU with partial pivoting, in place of A) beyond the n <= 256
+    kernel: n = 512, 4 RHS, batch 256, dense column-major, A ~ U(-1, 1) +
+    I/2: the AUTO blocked LU (register panels of 24 columns + gemm tile
+    updates) + getrs against the reference column sweeps (timed once).
+    Each step first restores A from a pristine copy (timed alone as
+    copy_ms and subtracted)."""
+    import torch
+
+    from blast_amd import capi
+    from blast_amd.layout import Layout
+
+    gen = torch.Generator(device="cuda").manual_seed(12)
+    b = torch.rand(batch, n, nrhs, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+
+    def pack(x, lay):
+        idx = torch.from_numpy(lay.index_map().reshape(-1)).cuda()
+        buf = torch.zeros(batch * lay.batch_stride, dtype=torch.float64, device="cuda")
+        buf.view(batch, lay.batch_stride).index_copy_(1, idx, x.reshape(batch, -1))
+        return capi.Operand(lay, buf)
+
     lc, lbc = Layout.make(n, n, panel=False), Layout.make(n, nrhs, panel=False)
     ag = torch.rand(batch, n, n, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
     ag += 0.5 * torch.eye(n, dtype=torch.float64, device="cuda")
@@ -250,8 +273,7 @@ def blocked_workload(steps, warmup, batch=256, n=512, nrhs=4):
         capi.set_algorithm(capi.ALGO_AUTO)
     g["reference_column_sweeps"] = {"ms_per_step_incl_copy": t_ref * 1e3, "solves_per_s": batch / (t_ref - t_copy),
                                     "max_solve_backward_error": bwd_g()}
-    out["dgesv"] = g
-    return out
+    return g
 
 
 def convert_workload(steps, warmup, batch=4096, n=128):
@@ -738,13 +760,14 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if args.only in ("gemm", "riccati", "mixed256", "adjoint", "convert", "blocked"):
+    if args.only in ("gemm", "riccati", "mixed256", "adjoint", "convert", "blocked", "blocked_lu"):
         res = {"gemm": lambda: gemm_workload(args.steps, args.warmup),
                "riccati": lambda: riccati_workload(args.steps, args.warmup, args.riccati_count),
                "mixed256": lambda: mixed256_workload(args.steps, args.warmup),
                "adjoint": lambda: adjoint_workload(args.steps, args.warmup),
                "convert": lambda: convert_workload(args.steps, args.warmup),
-               "blocked": lambda: blocked_workload(args.steps, args.warmup)}[args.only]()
+               "blocked": lambda: blocked_workload(args.steps, args.warmup),
+               "blocked_lu": lambda: blocked_lu_workload(args.steps, args.warmup)}[args.only]()
         print(json.dumps({args.only: res}))
         return
     batch = args.batch
