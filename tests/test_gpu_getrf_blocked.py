@@ -45,7 +45,7 @@ def run_getrf(la, ah, batch):
     return A.logical(batch), ipiv.cpu().numpy().reshape(batch, n), info.cpu().numpy()
 
 
-@pytest.mark.parametrize("n,panel", [(257, False), (300, True), (512, False), (700, True), (1024, False)])
+@pytest.mark.parametrize("n,panel", [(257, False), (257, True), (300, True), (512, False), (700, True), (1024, False)])
 def test_getrf_blocked_fp64_vs_oracle(n, panel):
     rng = np.random.default_rng(n)
     batch = 2
@@ -115,7 +115,7 @@ def test_getrf_blocked_same_pivot_row_twice_in_a_panel():
     np.testing.assert_allclose(lu, la.unpack(ref, 1), rtol=0, atol=1e-13 * np.abs(lu).max())
 
 
-@pytest.mark.parametrize("n,panel", [(384, False), (520, True)])
+@pytest.mark.parametrize("n,panel", [(384, False), (300, True), (520, True)])
 def test_getrf_blocked_fp32(n, panel):
     rng = np.random.default_rng(n + 1)
     batch = 2
