@@ -107,8 +107,9 @@ int potrf_blocked(hipStream_t s, Mat<T> A, Mat<T> L, int64_t batch, int32_t* inf
 
 // LU with partial pivoting, square 256 < n <= 1024, right-looking by
 // column blocks of the panel kernel's width (24 FP64, 32 FP32): the panel
-// kernel (factor, interchanges on the other columns, U12), then
-// A22 -= L21 U12 on the gemm tile kernel.  info is cleared first; an
+// kernel (factor, interchanges on the columns to its right, U12), then
+// A22 -= L21 U12 on the gemm tile kernel; the interchanges of the columns
+// left of each panel in one pass at the end.  info is cleared first; an
 // instance whose panel meets a zero pivot records it and its later panels do
 // nothing (BLAST stops there, dense/Getrf.hpp:52-119).
 template <typename T>
@@ -124,6 +125,7 @@ int getrf_blocked(hipStream_t s, Mat<T> A, int64_t* ipiv, int64_t ipiv_stride, i
                                      T(1), A22, A22, batch);
         }
     }
+    if (r == 0) r = launch_getrf_left_swaps<T>(s, A, ipiv, ipiv_stride, batch, info);
     return r;
 }
 

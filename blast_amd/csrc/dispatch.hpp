@@ -119,12 +119,18 @@ template <typename T> int launch_diag_info(hipStream_t, Mat<T> L, int64_t batch,
 
 // getrf_blocked.hip: the panel step of the blocked LU (square 256 < n <=
 // 1024, not transposed): factor columns k0..k0+nb-1 (nb <= the panel width) with partial
-// pivoting, apply the interchanges to the other columns and form U12.
+// pivoting, apply the interchanges to the columns right of the panel and
+// form U12.
 bool getrf_panel_covers(int m, int n, int trans);
 template <typename T> int getrf_panel_width();  // columns per panel (24 FP64, 32 FP32)
 template <typename T>
 int launch_getrf_panel(hipStream_t, Mat<T> A, int k0, int nb, int64_t* ipiv, int64_t ipiv_stride, int64_t batch,
                        int32_t* info);
+// after the last panel: the later panels' interchanges on the columns left
+// of each panel (skipped for instances with info != 0)
+template <typename T>
+int launch_getrf_left_swaps(hipStream_t, Mat<T> A, const int64_t* ipiv, int64_t ipiv_stride, int64_t batch,
+                            const int32_t* info);
 
 // dispatch.cpp
 template <typename T> int dispatch_gemm(hipStream_t, T, Mat<T>, Mat<T>, T, Mat<T>, Mat<T>, int64_t);

@@ -242,6 +242,9 @@ __global__ __launch_bounds__(kPt) void getrf_panel_kernel(Mat<T> Am, int k0, int
     int const done = fail ? fail : nb;  // columns whose pivot was chosen
     if (tid < done) ipiv[b * ipiv_stride + k0 + tid] = sm.piv[tid];
     if (fail) {
+        // the rest of the panel's pivots: no interchange (keeps ipiv a valid
+        // sequence when there is no info array to stop later panels)
+        if (tid >= done && tid < nb) ipiv[b * ipiv_stride + k0 + tid] = k0 + tid;
         if (info && tid == 0) info[b] = k0 + fail;
         return;
     }
@@ -254,9 +257,12 @@ __global__ __launch_bounds__(kPt) void getrf_panel_kernel(Mat<T> Am, int k0, int
     }
     __syncthreads();
 
-    // ---- the other columns: interchanges (gather, then scatter), and U12
-    for (int q = tid; q < N - nb; q += kPt) {
-        int const j = q < k0 ? q : q + nb, coff = ax.c(j);
+    // ---- the columns right of the panel: interchanges (gather, then
+    // scatter) and U12.  The columns left of it take the interchanges of all
+    // later panels in one pass at the end (getrf_left_swaps_kernel): no
+    // later step reads them.
+    for (int q = tid; q < N - k0 - nb; q += kPt) {
+        int const j = k0 + nb + q, coff = ax.c(j);
         // LDS reads indexed through an opaque zero: read again per column,
         // not hoisted out of the loop into (too many) registers
         int const z0 = opaque_s(0);
@@ -273,7 +279,7 @@ __global__ __launch_bounds__(kPt) void getrf_panel_kernel(Mat<T> Am, int k0, int
             x[c] = at(base, ax.r(rowat[cs]) + coff);
             z[c] = at(base, ax.r(rowat[pivs[cs] - k0]) + coff);
         }
-        if (j > k0) {
+        {
 #pragma unroll
             for (int c = 1; c < W; ++c) {
 #pragma unroll
@@ -288,6 +294,79 @@ __global__ __launch_bounds__(kPt) void getrf_panel_kernel(Mat<T> Am, int k0, int
         int const kx = opaque_s(k0);
 #pragma unroll
         for (int c = W - 1; c >= 0; --c) at(base, ax.r(kx + (c < nbq ? c : nbq - 1)) + coff) = x[c];
+    }
+}
+
+// The row interchanges of panels 1.. applied to the columns left of them,
+// after the last panel: for panel P (columns k0_P.., width W) every column
+// left of it needs the interchanges of panels P, P+1, ... in order.  From
+// the last panel down: sig = panel P's interchanges replayed on the identity
+// (thread 0, W swaps), g = sig o g (g: the row each row takes its content
+// from under panels P..last), then panel P - 1's columns take rows k0_P..n
+// through g, all loads before all stores.  Rows before k0_P never move.
+// Instances with info != 0 (a zero pivot stopped them) are left as they are.
+template <typename T>
+__global__ __launch_bounds__(kPt) void getrf_left_swaps_kernel(Mat<T> Am, int W, const int64_t* ipiv,
+                                                               int64_t ipiv_stride, const int32_t* info) {
+    constexpr int kCols = kPnb;  // columns of one panel in registers (W <= 32)
+    __shared__ int g[kPmax], sig[kPmax];
+    int64_t const b = blockIdx.x;
+    if (info && info[b] != 0) return;
+    T* const base = Am.inst(b);
+    Split<T> const ax{Am.spacing, Am.panel};
+    int const n = Am.rows, tid = threadIdx.x;
+    const int64_t* const piv = ipiv + b * ipiv_stride;
+    for (int r = tid; r < n; r += kPt) g[r] = r;
+    int const npan = (n + W - 1) / W;
+    for (int P = npan - 1; P >= 1; --P) {
+        int const kP = P * W, nbP = n - kP < W ? n - kP : W;
+        for (int r = kP + tid; r < n; r += kPt) sig[r] = r;
+        __syncthreads();
+        if (tid == 0) {
+            for (int c = 0; c < nbP; ++c) {
+                int const row = kP + c;
+                int64_t p = piv[row];
+                if (p < row || p >= n) p = row;  // (a malformed entry swaps nothing)
+                int const x = sig[row], y = sig[p];
+                sig[row] = y;
+                sig[p] = x;
+            }
+        }
+        __syncthreads();
+        for (int r = kP + tid; r < n; r += kPt) g[r] = sig[g[r]];
+        __syncthreads();
+        // panel P - 1's columns, rows kP..n-1 (at most 2 per thread)
+        // (offsets stepped one column at a time through an opaque add, and
+        // columns beyond W repeat the last one: no per-column scalar values
+        // or masks held live)
+        int const kQ = kP - W, cstep = ax.c(1), cq = ax.c(kQ);
+        T v[2][kCols];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            int const r = kP + tid + kPt * i;
+            int o = ax.r(r < n ? g[r] : kP) + cq;
+#pragma unroll
+            for (int c = 0; c < kCols; ++c) {
+                v[i][c] = at(base, o);
+                o = opaque_v(o + (c + 1 < W ? cstep : 0));
+            }
+        }
+        __syncthreads();  // every load of these columns before any store
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            int const r = kP + tid + kPt * i;
+            if (r < n) {
+                // stores in decreasing column order from column W - 1 (the
+                // repeats of it hold its own value)
+                int o = ax.r(r) + cq + (W - 1) * cstep;
+#pragma unroll
+                for (int c = kCols - 1; c >= 0; --c) {
+                    at(base, o) = v[i][c];
+                    o = opaque_v(o - (c <= W - 1 ? cstep : 0));
+                }
+            }
+        }
+        __syncthreads();  // (sig is rebuilt next)
     }
 }
 
@@ -312,6 +391,23 @@ int launch_getrf_panel(hipStream_t s, Mat<T> A, int k0, int nb, int64_t* ipiv, i
     return check_launch("getrf_panel_kernel");
 }
 
+template <typename T>
+int launch_getrf_left_swaps(hipStream_t s, Mat<T> A, const int64_t* ipiv, int64_t ipiv_stride, int64_t batch,
+                            const int32_t* info) {
+    if (batch == 0 || A.rows <= kPw<T>) return 0;
+    if (A.trans || A.rows > kPmax || A.rows != A.cols) {
+        set_error("getrf_left_swaps: shape not covered (%d x %d)", A.rows, A.cols);
+        return BLAST_HIP_ERR_RUNTIME;
+    }
+    hipLaunchKernelGGL((getrf_left_swaps_kernel<T>), dim3(unsigned(batch)), dim3(kPt), 0, s, A, kPw<T>, ipiv,
+                       ipiv_stride, info);
+    return check_launch("getrf_left_swaps_kernel");
+}
+
+template int launch_getrf_left_swaps<double>(hipStream_t, Mat<double>, const int64_t*, int64_t, int64_t,
+                                             const int32_t*);
+template int launch_getrf_left_swaps<float>(hipStream_t, Mat<float>, const int64_t*, int64_t, int64_t,
+                                            const int32_t*);
 template int getrf_panel_width<double>();
 template int getrf_panel_width<float>();
 template int launch_getrf_panel<double>(hipStream_t, Mat<double>, int, int, int64_t*, int64_t, int64_t, int32_t*);

@@ -29,6 +29,7 @@ RULES = {
     "posv_uneven_kernel": dict(scratch=0, vgpr_spill=0, calls=False),
     "posv_backward_uneven_kernel": dict(scratch=0, vgpr_spill=0, calls=False),
     "getrf_panel_kernel": dict(scratch=0, vgpr_spill=0, sgpr_spill=0, calls=False),
+    "getrf_left_swaps_kernel": dict(scratch=0, vgpr_spill=0, sgpr_spill=0, calls=False),
     "sposv_ir_kernel": dict(scratch=0, vgpr_spill=0, calls=True),
     "riccati_persistent_kernel": dict(scratch=12, vgpr_spill=0, calls=True),
 }
