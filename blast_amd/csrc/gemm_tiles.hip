@@ -1,8 +1,8 @@
 // Batched D = alpha*op(A)*op(B) + beta*C on MFMA tiles (FP64: v_mfma_f64_16x16x4f64,
-// FP32: v_mfma_f32_16x16x4f32), 256-thread workgroups: ceil(strips / 4) per
-// instance (grid y), so that each wavefront owns one 16-row strip of D.
+// FP32: v_mfma_f32_16x16x4f32), 256-thread workgroups, grid y = the 16 x 64
+// blocks of D in groups of four (one block per wavefront).
 //
-// Each wavefront owns 16-row strips of D (strip R = wavefront + 4 y, ...) and
+// Each wavefront owns 16-row strips of D (strip R = wavefront + 4 rs, ...) and
 // computes up to four 16x16 tiles of a strip at once: per K-chunk of 4 it
 // loads one A-operand element per lane (reused across the four tiles) and one
 // B-operand element per tile.  The epilogue is BLAST's: d = fma(beta, c,
@@ -26,7 +26,7 @@ enum GemmMode : int { kGemm = 0, kSyrkLower = 1, kTrmmLeftUpper = 2, kTrmmRightL
 
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void gemm_tile_kernel(T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D,
-                                                        int K) {
+                                                        int K, unsigned ts_count) {
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
     int64_t const b = blockIdx.x;
@@ -37,11 +37,15 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(T alpha, Mat<T> A, Mat<T
     int const MT = (Mr + 15) / 16, NTt = (Nc + 15) / 16, KQ = (K + 3) / 4;
     int const m = lane & 15, g = lane >> 4;
 
-    // row strips R = w + 4 blockIdx.y, + 4 gridDim.y, ...: large outputs
-    // spread over several workgroups per instance (occupancy)
-    for (int R = w + 4 * int(blockIdx.y); R < MT; R += 4 * int(gridDim.y)) {
+    // Large outputs spread over the workgroups of grid y = RS x TS: strips
+    // R = w + 4 rs, + 4 RS, ..., column groups T0 = 4 ts, + 4 TS, ... (one
+    // 16 x 64 block per wavefront when the grid covers the output; the many
+    // short wavefronts hide each other's memory latency)
+    int const TS = int(ts_count), RS = int(gridDim.y) / TS;
+    int const rs = int(blockIdx.y) / TS, ts = int(blockIdx.y) % TS;
+    for (int R = w + 4 * rs; R < MT; R += 4 * RS) {
         int const row = 16 * R + m;  // A-operand row of this lane
-        for (int T0 = 0; T0 < NTt; T0 += 4) {
+        for (int T0 = 4 * ts; T0 < NTt; T0 += 4 * TS) {
             if (MODE == kSyrkLower && 16 * T0 > 16 * R + 15) break;  // strictly upper tiles
             acc_t acc[4];
 #pragma unroll
@@ -172,10 +176,13 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(T alpha, Mat<T> A, Mat<
 template <typename T, int MODE>
 static int launch_mode(hipStream_t s, T alpha, Mat<T> A, Mat<T> B, T beta, Mat<T> C, Mat<T> D, int K, int64_t batch) {
     if (batch == 0 || D.rows == 0 || D.cols == 0) return 0;
-    // one row strip per wavefront: ceil(strips / 4) workgroups per instance
-    int const slices = (((D.rows + 15) / 16) + 3) / 4;
-    hipLaunchKernelGGL((gemm_tile_kernel<T, MODE>), dim3(unsigned(batch), unsigned(slices)), dim3(256), 0, s, alpha, A,
-                       B, beta, C, D, K);
+    // one 16 x 64 block per wavefront: RS x TS workgroups per instance
+    // (grid y at most 65535; beyond that the wavefronts loop)
+    unsigned const RS = unsigned((((D.rows + 15) / 16) + 3) / 4), TSn = unsigned((((D.cols + 15) / 16) + 3) / 4);
+    unsigned const TS = TSn < 65535u / RS ? TSn : (65535u / RS > 0 ? 65535u / RS : 1u);
+    unsigned const RSc = RS < 65535u ? RS : 65535u;
+    hipLaunchKernelGGL((gemm_tile_kernel<T, MODE>), dim3(unsigned(batch), RSc * TS), dim3(256), 0, s, alpha, A, B,
+                       beta, C, D, K, TS);
     return check_launch("gemm_tile_kernel");
 }
 
