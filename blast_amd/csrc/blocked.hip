@@ -9,16 +9,32 @@ namespace blast_hip {
 namespace {
 
 constexpr int kBl = 256;
+constexpr int kCopyCols = 8;  // columns per workgroup of copy_lower_kernel
 
-// L(i, j) = A(i, j) for i >= j (the rest of L untouched)
+// L(i, j) = A(i, j) for i >= j (the rest of L untouched); workgroup
+// (instance, y) copies 8 panels (panel storage) or 8 columns (otherwise),
+// consecutive addresses per thread
 template <typename T>
 __global__ __launch_bounds__(kBl) void copy_lower_kernel(Mat<T> Am, Mat<T> Lm) {
     Ref<T> a(Am, blockIdx.x), l(Lm, blockIdx.x);
     int const n = Am.rows;
-    for (int e = threadIdx.x; e < n * n; e += kBl) {
-        int const i = e % n, j = e / n;
-        if (i >= j) l(i, j) = a(i, j);
+    if (Am.panel && Lm.panel && !Am.trans && !Lm.trans) {
+        // panel storage: workgroup y copies panels 8y..8y+7 (PS rows each,
+        // stored contiguously column after column) in storage order
+        constexpr int PS = PanelSize<T>::value, LOG = PanelSize<T>::log2;
+        int const p0 = kCopyCols * int(blockIdx.y), np = (n + PS - 1) / PS;
+        for (int p = p0; p < p0 + kCopyCols && p < np; ++p) {
+            int const jmax = PS * p + PS < n ? PS * p + PS : n;  // columns 0..jmax-1 meet the panel's lower part
+            for (int e = int(threadIdx.x); e < PS * jmax; e += kBl) {
+                int const i = PS * p + (e & (PS - 1)), j = e >> LOG;
+                if (i < n && i >= j) l(i, j) = a(i, j);
+            }
+        }
+        return;
     }
+    int const j0 = kCopyCols * int(blockIdx.y);  // columns 8y..8y+7, rows down each column
+    for (int j = j0; j < j0 + kCopyCols && j < n; ++j)
+        for (int i = j + int(threadIdx.x); i < n; i += kBl) l(i, j) = a(i, j);
 }
 
 // X = alpha B (same logical shape)
@@ -55,7 +71,9 @@ __global__ __launch_bounds__(kBl) void diag_info_kernel(Mat<T> Lm, int32_t* info
 template <typename T>
 int launch_copy_lower(hipStream_t s, Mat<T> A, Mat<T> L, int64_t batch) {
     if (batch == 0) return 0;
-    hipLaunchKernelGGL((copy_lower_kernel<T>), dim3(unsigned(batch)), dim3(kBl), 0, s, A, L);
+    unsigned const groups = unsigned((A.rows + kCopyCols - 1) / kCopyCols);
+    if (groups == 0) return 0;
+    hipLaunchKernelGGL((copy_lower_kernel<T>), dim3(unsigned(batch), groups), dim3(kBl), 0, s, A, L);
     return check_launch("copy_lower_kernel");
 }
 template <typename T>
