@@ -72,12 +72,137 @@ __device__ __forceinline__ void adj_store_block(double* s, double const (&reg)[k
     }
 }
 
+// One instance: the two block solves on the right-hand sides in LDS and the
+// outputs, for R >= r right-hand-side columns (R = 1, 2, 4, 8: the row
+// updates and the diagonal solves cover R columns, not all 8).
+template <int R>
+__device__ __forceinline__ void adj_instance(AdjLds& sm, Ref<double> const& L, Ref<double> const& X,
+                                             Ref<double> const& XB, Ref<double> const& AB, Ref<double> const& BB,
+                                             int n, int r) {
+    int const tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int const li = lane & 15, lc = lane >> 4;  // wavefront 0's diagonal solve: row li, right-hand side lc (+4)
+    int const NT = (n + 15) / 16, npad = 16 * NT;
+    double reg[kAdjPer];
+    if (NT > 0) adj_load_block<true>(L, n, 0, reg);
+    for (int e = tid; e < npad * kAdjRhs; e += kAdjT) {
+        int const row = e / kAdjRhs, c = e % kAdjRhs;
+        bool const in = row < n && c < r;
+        sm.v[e] = in ? XB(row, c) : 0.0;
+        sm.x[e] = in ? X(row, c) : 0.0;
+    }
+    constexpr int H = (R + 3) / 4;  // diagonal-solve passes (4 columns each)
+    // ---- y = L^{-1} Xbar, by 16-row blocks
+    for (int kb = 0; kb < NT; ++kb) {
+        int const r0 = 16 * kb;
+        __syncthreads();  // the previous block's update has read sm.l
+        adj_store_block(sm.l, reg);
+        __syncthreads();
+        if (kb + 1 < NT)
+            adj_load_block<true>(L, n, kb + 1, reg);
+        else if (NT > 0)
+            adj_load_block<false>(L, n, NT - 1, reg);  // the backward solve's first block row
+        if (w == 0) {
+#pragma unroll
+            for (int h = 0; h < H; ++h) {
+                int const c = lc + 4 * h;
+                if (c < R) {
+                    double y = sm.v[(r0 + li) * kAdjRhs + c];
+                    static_for<0, 16>([&](auto J) __attribute__((always_inline)) {
+                        constexpr int j = decltype(J)::value;
+                        if (li == j) y = y / sm.l[j * kAdjLs + j];
+                        double const yj = row_bcast<j>(y);
+                        if (li > j) y = __builtin_fma(-sm.l[li * kAdjLs + j], yj, y);
+                    });
+                    sm.v[(r0 + li) * kAdjRhs + c] = y;
+                }
+            }
+        }
+        __syncthreads();
+        // rows below the block: v_i -= L_i,kb y_kb
+        for (int i = r0 + 16 + tid; i < npad; i += kAdjT) {
+            double const* const lrow = sm.l + (i - r0) * kAdjLs;
+            double acc[R];
+#pragma unroll
+            for (int c = 0; c < R; ++c) acc[c] = sm.v[i * kAdjRhs + c];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                double const lk = lrow[k];
+#pragma unroll
+                for (int c = 0; c < R; ++c) acc[c] = __builtin_fma(-lk, sm.v[(r0 + k) * kAdjRhs + c], acc[c]);
+            }
+#pragma unroll
+            for (int c = 0; c < R; ++c) sm.v[i * kAdjRhs + c] = acc[c];
+        }
+    }
+    // ---- Bbar = L^{-T} y, by 16-row blocks from the bottom
+    for (int kb = NT - 1; kb >= 0; --kb) {
+        int const r0 = 16 * kb;
+        __syncthreads();
+        adj_store_block(sm.l, reg);  // element (r0 + k, j) at [j * 17 + k]
+        __syncthreads();
+        if (kb > 0) adj_load_block<false>(L, n, kb - 1, reg);
+        if (w == 0) {
+            // L_kk^T z = v (upper): z_j final from the bottom, rows i < j apply it
+#pragma unroll
+            for (int h = 0; h < H; ++h) {
+                int const c = lc + 4 * h;
+                if (c < R) {
+                    double z = sm.v[(r0 + li) * kAdjRhs + c];
+                    static_for<0, 16>([&](auto JJ) __attribute__((always_inline)) {
+                        constexpr int j = 15 - decltype(JJ)::value;
+                        if (li == j) z = z / sm.l[(r0 + j) * kAdjLs + j];
+                        double const zj = row_bcast<j>(z);
+                        if (li < j) z = __builtin_fma(-sm.l[(r0 + li) * kAdjLs + j], zj, z);
+                    });
+                    sm.v[(r0 + li) * kAdjRhs + c] = z;
+                }
+            }
+        }
+        __syncthreads();
+        // rows above the block: v_i -= sum_k L(r0 + k, i) z_k
+        for (int i = tid; i < r0; i += kAdjT) {
+            double const* const lcol = sm.l + i * kAdjLs;
+            double acc[R];
+#pragma unroll
+            for (int c = 0; c < R; ++c) acc[c] = sm.v[i * kAdjRhs + c];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                double const lk = lcol[k];
+#pragma unroll
+                for (int c = 0; c < R; ++c) acc[c] = __builtin_fma(-lk, sm.v[(r0 + k) * kAdjRhs + c], acc[c]);
+            }
+#pragma unroll
+            for (int c = 0; c < R; ++c) sm.v[i * kAdjRhs + c] = acc[c];
+        }
+    }
+    __syncthreads();
+    // ---- outputs: Bbar, and Abar = -fold(Bbar X^T) over the whole n x n
+    for (int e = tid; e < n * r; e += kAdjT) {
+        int const row = e % n, c = e / n;
+        BB(row, c) = sm.v[row * kAdjRhs + c];
+    }
+    for (int e = tid; e < n * n; e += kAdjT) {
+        int const i = e % n, j = e / n;
+        double v = 0.0;
+        if (i > j) {
+#pragma unroll
+            for (int c = 0; c < R; ++c)
+                if (c < r)
+                    v = v - (sm.v[i * kAdjRhs + c] * sm.x[j * kAdjRhs + c] + sm.v[j * kAdjRhs + c] * sm.x[i * kAdjRhs + c]);
+        } else if (i == j) {
+#pragma unroll
+            for (int c = 0; c < R; ++c)
+                if (c < r) v = v - sm.v[i * kAdjRhs + c] * sm.x[i * kAdjRhs + c];
+        }
+        AB(i, j) = v;
+    }
+}
+
 __global__ __launch_bounds__(kAdjT, 2) void posv_backward_uneven_kernel(const blast_adjoint_instance_t* inst,
                                                                         int64_t count, int32_t* queue, int32_t* info) {
     __shared__ AdjLds sm;
     __shared__ int sIdx;
-    int const tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    int const li = lane & 15, lc = lane >> 4;  // wavefront 0's diagonal solve: row li, right-hand side lc (+4)
+    int const tid = threadIdx.x;
     for (;;) {
         if (tid == 0) sIdx = atomicAdd(&queue[0], 1);
         __syncthreads();
@@ -94,112 +219,16 @@ __global__ __launch_bounds__(kAdjT, 2) void posv_backward_uneven_kernel(const bl
         Mat<double> const Lm = Mat<double>::from(d.l), Xm = Mat<double>::from(d.x), XBm = Mat<double>::from(d.xb);
         Mat<double> const ABm = Mat<double>::from(d.ab), BBm = Mat<double>::from(d.bb);
         Ref<double> const L(Lm, 0), X(Xm, 0), XB(XBm, 0), AB(ABm, 0), BB(BBm, 0);
-        int const n = int(d.l.rows), r = int(d.x.cols), NT = (n + 15) / 16, npad = 16 * NT;
-        double reg[kAdjPer];
-        if (NT > 0) adj_load_block<true>(L, n, 0, reg);
-        for (int e = tid; e < npad * kAdjRhs; e += kAdjT) {
-            int const row = e / kAdjRhs, c = e % kAdjRhs;
-            bool const in = row < n && c < r;
-            sm.v[e] = in ? XB(row, c) : 0.0;
-            sm.x[e] = in ? X(row, c) : 0.0;
-        }
-        // ---- y = L^{-1} Xbar, by 16-row blocks
-        for (int kb = 0; kb < NT; ++kb) {
-            int const r0 = 16 * kb;
-            __syncthreads();  // the previous block's update has read sm.l
-            adj_store_block(sm.l, reg);
-            __syncthreads();
-            if (kb + 1 < NT)
-                adj_load_block<true>(L, n, kb + 1, reg);
-            else if (NT > 0)
-                adj_load_block<false>(L, n, NT - 1, reg);  // the backward solve's first block row
-            if (w == 0) {
-#pragma unroll
-                for (int h = 0; h < kAdjRhs / 4; ++h) {
-                    int const c = lc + 4 * h;
-                    double y = sm.v[(r0 + li) * kAdjRhs + c];
-                    static_for<0, 16>([&](auto J) __attribute__((always_inline)) {
-                        constexpr int j = decltype(J)::value;
-                        if (li == j) y = y / sm.l[j * kAdjLs + j];
-                        double const yj = row_bcast<j>(y);
-                        if (li > j) y = __builtin_fma(-sm.l[li * kAdjLs + j], yj, y);
-                    });
-                    sm.v[(r0 + li) * kAdjRhs + c] = y;
-                }
-            }
-            __syncthreads();
-            // rows below the block: v_i -= L_i,kb y_kb
-            for (int i = r0 + 16 + tid; i < npad; i += kAdjT) {
-                double const* const lrow = sm.l + (i - r0) * kAdjLs;
-                double acc[kAdjRhs];
-#pragma unroll
-                for (int c = 0; c < kAdjRhs; ++c) acc[c] = sm.v[i * kAdjRhs + c];
-#pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    double const lk = lrow[k];
-#pragma unroll
-                    for (int c = 0; c < kAdjRhs; ++c) acc[c] = __builtin_fma(-lk, sm.v[(r0 + k) * kAdjRhs + c], acc[c]);
-                }
-#pragma unroll
-                for (int c = 0; c < kAdjRhs; ++c) sm.v[i * kAdjRhs + c] = acc[c];
-            }
-        }
-        // ---- Bbar = L^{-T} y, by 16-row blocks from the bottom
-        for (int kb = NT - 1; kb >= 0; --kb) {
-            int const r0 = 16 * kb;
-            __syncthreads();
-            adj_store_block(sm.l, reg);  // element (r0 + k, j) at [j * 17 + k]
-            __syncthreads();
-            if (kb > 0) adj_load_block<false>(L, n, kb - 1, reg);
-            if (w == 0) {
-                // L_kk^T z = v (upper): z_j final from the bottom, rows i < j apply it
-#pragma unroll
-                for (int h = 0; h < kAdjRhs / 4; ++h) {
-                    int const c = lc + 4 * h;
-                    double z = sm.v[(r0 + li) * kAdjRhs + c];
-                    static_for<0, 16>([&](auto JJ) __attribute__((always_inline)) {
-                        constexpr int j = 15 - decltype(JJ)::value;
-                        if (li == j) z = z / sm.l[(r0 + j) * kAdjLs + j];
-                        double const zj = row_bcast<j>(z);
-                        if (li < j) z = __builtin_fma(-sm.l[(r0 + li) * kAdjLs + j], zj, z);
-                    });
-                    sm.v[(r0 + li) * kAdjRhs + c] = z;
-                }
-            }
-            __syncthreads();
-            // rows above the block: v_i -= sum_k L(r0 + k, i) z_k
-            for (int i = tid; i < r0; i += kAdjT) {
-                double const* const lcol = sm.l + i * kAdjLs;
-                double acc[kAdjRhs];
-#pragma unroll
-                for (int c = 0; c < kAdjRhs; ++c) acc[c] = sm.v[i * kAdjRhs + c];
-#pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    double const lk = lcol[k];
-#pragma unroll
-                    for (int c = 0; c < kAdjRhs; ++c) acc[c] = __builtin_fma(-lk, sm.v[(r0 + k) * kAdjRhs + c], acc[c]);
-                }
-#pragma unroll
-                for (int c = 0; c < kAdjRhs; ++c) sm.v[i * kAdjRhs + c] = acc[c];
-            }
-        }
-        __syncthreads();
-        // ---- outputs: Bbar, and Abar = -fold(Bbar X^T) over the whole n x n
-        for (int e = tid; e < n * r; e += kAdjT) {
-            int const row = e % n, c = e / n;
-            BB(row, c) = sm.v[row * kAdjRhs + c];
-        }
-        for (int e = tid; e < n * n; e += kAdjT) {
-            int const i = e % n, j = e / n;
-            double v = 0.0;
-            if (i > j) {
-                for (int c = 0; c < r; ++c)
-                    v = v - (sm.v[i * kAdjRhs + c] * sm.x[j * kAdjRhs + c] + sm.v[j * kAdjRhs + c] * sm.x[i * kAdjRhs + c]);
-            } else if (i == j) {
-                for (int c = 0; c < r; ++c) v = v - sm.v[i * kAdjRhs + c] * sm.x[i * kAdjRhs + c];
-            }
-            AB(i, j) = v;
-        }
+        int const n = int(d.l.rows), r = int(d.x.cols);
+        // (uniform: one descriptor per workgroup)
+        if (r <= 1)
+            adj_instance<1>(sm, L, X, XB, AB, BB, n, r);
+        else if (r <= 2)
+            adj_instance<2>(sm, L, X, XB, AB, BB, n, r);
+        else if (r <= 4)
+            adj_instance<4>(sm, L, X, XB, AB, BB, n, r);
+        else
+            adj_instance<8>(sm, L, X, XB, AB, BB, n, r);
         if (info && tid == 0) info[idx] = 0;
     }
 }
