@@ -14,9 +14,11 @@
 // column of L below and on the diagonal (forward) or the block row left of
 // and on it (backward) is staged into LDS, wavefront 0 solves the 16x16
 // diagonal system (one lane per row and right-hand side, the finished value
-// broadcast within the 16-lane row by DPP), and every thread then updates
-// one row of the remaining right-hand sides from the staged block.  The next
-// block's L is loaded into registers while the current one is used.
+// broadcast within the 16-lane row by DPP; the diagonal's reciprocals formed
+// first, so the dependent chain multiplies), and every thread then updates
+// one row of the remaining right-hand sides from the staged block (loops
+// specialised to 1, 2, 4 or 8 right-hand sides).  Block loads are issued two
+// steps ahead (two register sets).
 // Algorithm references: the potrf/trsm adjoints of BLAST's dense kernels are
 // not in BLAST; the formulas are the standard ones (Giles 2008, Murray 2016)
 // the uniform entry points and oracle/adjoint.py use.
@@ -82,8 +84,17 @@ __device__ __forceinline__ void adj_instance(AdjLds& sm, Ref<double> const& L, R
     int const tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     int const li = lane & 15, lc = lane >> 4;  // wavefront 0's diagonal solve: row li, right-hand side lc (+4)
     int const NT = (n + 15) / 16, npad = 16 * NT;
-    double reg[kAdjPer];
-    if (NT > 0) adj_load_block<true>(L, n, 0, reg);
+    // block loads of step st (forward block column st, then backward block
+    // row 2NT-1-st); nothing beyond the last step
+    auto load_step = [&](int st, double (&reg)[kAdjPer]) __attribute__((always_inline)) {
+        if (st < NT)
+            adj_load_block<true>(L, n, st, reg);
+        else if (st < 2 * NT)
+            adj_load_block<false>(L, n, 2 * NT - 1 - st, reg);
+    };
+    double rA[kAdjPer], rB[kAdjPer];
+    load_step(0, rA);
+    load_step(1, rB);
     for (int e = tid; e < npad * kAdjRhs; e += kAdjT) {
         int const row = e / kAdjRhs, c = e % kAdjRhs;
         bool const in = row < n && c < r;
@@ -91,89 +102,93 @@ __device__ __forceinline__ void adj_instance(AdjLds& sm, Ref<double> const& L, R
         sm.x[e] = in ? X(row, c) : 0.0;
     }
     constexpr int H = (R + 3) / 4;  // diagonal-solve passes (4 columns each)
-    // ---- y = L^{-1} Xbar, by 16-row blocks
-    for (int kb = 0; kb < NT; ++kb) {
-        int const r0 = 16 * kb;
-        __syncthreads();  // the previous block's update has read sm.l
+    // ---- the block steps: s = 0..NT-1 the forward solve y = L^{-1} Xbar by
+    // 16-row blocks (block column s of L staged), s = NT..2NT-1 the
+    // backward solve Bbar = L^{-T} y from the bottom (block row 2NT-1-s).
+    // Two register sets alternate (the step pair is unrolled), so each
+    // block's loads are issued two steps before it is staged.
+    auto step = [&](int st, double (&reg)[kAdjPer]) __attribute__((always_inline)) {
+        __syncthreads();  // the previous step's reads of sm.l are done
         adj_store_block(sm.l, reg);
         __syncthreads();
-        if (kb + 1 < NT)
-            adj_load_block<true>(L, n, kb + 1, reg);
-        else if (NT > 0)
-            adj_load_block<false>(L, n, NT - 1, reg);  // the backward solve's first block row
-        if (w == 0) {
+        load_step(st + 2, reg);
+        if (st < NT) {
+            int const kb = st, r0 = 16 * kb;
+            if (w == 0) {
+                // 1 / l_jj formed for all 16 rows at once, off the chain
+                double const dinv = 1.0 / sm.l[li * kAdjLs + li];
 #pragma unroll
-            for (int h = 0; h < H; ++h) {
-                int const c = lc + 4 * h;
-                if (c < R) {
-                    double y = sm.v[(r0 + li) * kAdjRhs + c];
-                    static_for<0, 16>([&](auto J) __attribute__((always_inline)) {
-                        constexpr int j = decltype(J)::value;
-                        if (li == j) y = y / sm.l[j * kAdjLs + j];
-                        double const yj = row_bcast<j>(y);
-                        if (li > j) y = __builtin_fma(-sm.l[li * kAdjLs + j], yj, y);
-                    });
-                    sm.v[(r0 + li) * kAdjRhs + c] = y;
+                for (int h = 0; h < H; ++h) {
+                    int const c = lc + 4 * h;
+                    if (c < R) {
+                        double y = sm.v[(r0 + li) * kAdjRhs + c];
+                        static_for<0, 16>([&](auto J) __attribute__((always_inline)) {
+                            constexpr int j = decltype(J)::value;
+                            if (li == j) y = y * dinv;
+                            double const yj = row_bcast<j>(y);
+                            if (li > j) y = __builtin_fma(-sm.l[li * kAdjLs + j], yj, y);
+                        });
+                        sm.v[(r0 + li) * kAdjRhs + c] = y;
+                    }
                 }
             }
-        }
-        __syncthreads();
-        // rows below the block: v_i -= L_i,kb y_kb
-        for (int i = r0 + 16 + tid; i < npad; i += kAdjT) {
-            double const* const lrow = sm.l + (i - r0) * kAdjLs;
-            double acc[R];
+            __syncthreads();
+            // rows below the block: v_i -= L_i,kb y_kb
+            for (int i = r0 + 16 + tid; i < npad; i += kAdjT) {
+                double const* const lrow = sm.l + (i - r0) * kAdjLs;
+                double acc[R];
 #pragma unroll
-            for (int c = 0; c < R; ++c) acc[c] = sm.v[i * kAdjRhs + c];
+                for (int c = 0; c < R; ++c) acc[c] = sm.v[i * kAdjRhs + c];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                double const lk = lrow[k];
+                for (int k = 0; k < 16; ++k) {
+                    double const lk = lrow[k];
 #pragma unroll
-                for (int c = 0; c < R; ++c) acc[c] = __builtin_fma(-lk, sm.v[(r0 + k) * kAdjRhs + c], acc[c]);
+                    for (int c = 0; c < R; ++c) acc[c] = __builtin_fma(-lk, sm.v[(r0 + k) * kAdjRhs + c], acc[c]);
+                }
+#pragma unroll
+                for (int c = 0; c < R; ++c) sm.v[i * kAdjRhs + c] = acc[c];
             }
+        } else {
+            int const kb = 2 * NT - 1 - st, r0 = 16 * kb;
+            if (w == 0) {
+                // L_kk^T z = v (upper): z_j final from the bottom, rows i < j apply it
+                double const dinv = 1.0 / sm.l[(r0 + li) * kAdjLs + li];
 #pragma unroll
-            for (int c = 0; c < R; ++c) sm.v[i * kAdjRhs + c] = acc[c];
-        }
-    }
-    // ---- Bbar = L^{-T} y, by 16-row blocks from the bottom
-    for (int kb = NT - 1; kb >= 0; --kb) {
-        int const r0 = 16 * kb;
-        __syncthreads();
-        adj_store_block(sm.l, reg);  // element (r0 + k, j) at [j * 17 + k]
-        __syncthreads();
-        if (kb > 0) adj_load_block<false>(L, n, kb - 1, reg);
-        if (w == 0) {
-            // L_kk^T z = v (upper): z_j final from the bottom, rows i < j apply it
-#pragma unroll
-            for (int h = 0; h < H; ++h) {
-                int const c = lc + 4 * h;
-                if (c < R) {
-                    double z = sm.v[(r0 + li) * kAdjRhs + c];
-                    static_for<0, 16>([&](auto JJ) __attribute__((always_inline)) {
-                        constexpr int j = 15 - decltype(JJ)::value;
-                        if (li == j) z = z / sm.l[(r0 + j) * kAdjLs + j];
-                        double const zj = row_bcast<j>(z);
-                        if (li < j) z = __builtin_fma(-sm.l[(r0 + li) * kAdjLs + j], zj, z);
-                    });
-                    sm.v[(r0 + li) * kAdjRhs + c] = z;
+                for (int h = 0; h < H; ++h) {
+                    int const c = lc + 4 * h;
+                    if (c < R) {
+                        double z = sm.v[(r0 + li) * kAdjRhs + c];
+                        static_for<0, 16>([&](auto JJ) __attribute__((always_inline)) {
+                            constexpr int j = 15 - decltype(JJ)::value;
+                            if (li == j) z = z * dinv;
+                            double const zj = row_bcast<j>(z);
+                            if (li < j) z = __builtin_fma(-sm.l[(r0 + li) * kAdjLs + j], zj, z);
+                        });
+                        sm.v[(r0 + li) * kAdjRhs + c] = z;
+                    }
                 }
             }
-        }
-        __syncthreads();
-        // rows above the block: v_i -= sum_k L(r0 + k, i) z_k
-        for (int i = tid; i < r0; i += kAdjT) {
-            double const* const lcol = sm.l + i * kAdjLs;
-            double acc[R];
+            __syncthreads();
+            // rows above the block: v_i -= sum_k L(r0 + k, i) z_k
+            for (int i = tid; i < r0; i += kAdjT) {
+                double const* const lcol = sm.l + i * kAdjLs;
+                double acc[R];
 #pragma unroll
-            for (int c = 0; c < R; ++c) acc[c] = sm.v[i * kAdjRhs + c];
+                for (int c = 0; c < R; ++c) acc[c] = sm.v[i * kAdjRhs + c];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                double const lk = lcol[k];
+                for (int k = 0; k < 16; ++k) {
+                    double const lk = lcol[k];
 #pragma unroll
-                for (int c = 0; c < R; ++c) acc[c] = __builtin_fma(-lk, sm.v[(r0 + k) * kAdjRhs + c], acc[c]);
+                    for (int c = 0; c < R; ++c) acc[c] = __builtin_fma(-lk, sm.v[(r0 + k) * kAdjRhs + c], acc[c]);
+                }
+#pragma unroll
+                for (int c = 0; c < R; ++c) sm.v[i * kAdjRhs + c] = acc[c];
             }
-#pragma unroll
-            for (int c = 0; c < R; ++c) sm.v[i * kAdjRhs + c] = acc[c];
         }
+    };
+    for (int st = 0; st < 2 * NT; st += 2) {
+        step(st, rA);
+        step(st + 1, rB);
     }
     __syncthreads();
     // ---- outputs: Bbar, and Abar = -fold(Bbar X^T) over the whole n x n
@@ -181,8 +196,11 @@ __device__ __forceinline__ void adj_instance(AdjLds& sm, Ref<double> const& L, R
         int const row = e % n, c = e / n;
         BB(row, c) = sm.v[row * kAdjRhs + c];
     }
+    // element e = tid + 256 t at (i, j) = (e mod n, e div n), stepped
+    // without a division per element
+    int const si = kAdjT % (n > 0 ? n : 1), sj = kAdjT / (n > 0 ? n : 1);
+    int i = tid % (n > 0 ? n : 1), j = tid / (n > 0 ? n : 1);
     for (int e = tid; e < n * n; e += kAdjT) {
-        int const i = e % n, j = e / n;
         double v = 0.0;
         if (i > j) {
 #pragma unroll
@@ -195,6 +213,12 @@ __device__ __forceinline__ void adj_instance(AdjLds& sm, Ref<double> const& L, R
                 if (c < r) v = v - sm.v[i * kAdjRhs + c] * sm.x[i * kAdjRhs + c];
         }
         AB(i, j) = v;
+        i += si;
+        j += sj;
+        if (i >= n) {
+            i -= n;
+            ++j;
+        }
     }
 }
 
