@@ -51,20 +51,22 @@ __global__ __launch_bounds__(kEw) void fold_lower_kernel(Mat<T> Gm) {
 
 // Abar := -(Bbar X^T + X Bbar^T) folded onto lower entries, i.e.
 // -(sum_c bb_ic x_jc + bb_jc x_ic) below the diagonal, -sum_c bb_ic x_ic on it.
+// Workgroup (instance, y) writes columns 8y..8y+7, one row per thread.
+constexpr int kFoldCols = 8;
 template <typename T>
 __global__ __launch_bounds__(kEw) void sym_outer_lower_kernel(Mat<T> Bb, Mat<T> Xm, Mat<T> Ab) {
     int64_t const b = blockIdx.x;
     Ref<T> bb(Bb, b), x(Xm, b), a(Ab, b);
-    int const n = Ab.rows, r = Xm.cols;
-    for (int e = threadIdx.x; e < n * n; e += kEw) {
-        int const i = e % n, j = e / n;
-        T v = T(0);
-        if (i > j)
-            for (int c = 0; c < r; ++c) v = v - (bb(i, c) * x(j, c) + bb(j, c) * x(i, c));
-        else if (i == j)
-            for (int c = 0; c < r; ++c) v = v - bb(i, c) * x(i, c);
-        a(i, j) = v;
-    }
+    int const n = Ab.rows, r = Xm.cols, j0 = kFoldCols * int(blockIdx.y);
+    for (int i = int(threadIdx.x); i < n; i += kEw)
+        for (int j = j0; j < j0 + kFoldCols && j < n; ++j) {
+            T v = T(0);
+            if (i > j)
+                for (int c = 0; c < r; ++c) v = v - (bb(i, c) * x(j, c) + bb(j, c) * x(i, c));
+            else if (i == j)
+                for (int c = 0; c < r; ++c) v = v - bb(i, c) * x(i, c);
+            a(i, j) = v;
+        }
 }
 
 // Zero the strict upper (upper = 0) or strict lower (upper = 1) triangle.
@@ -187,7 +189,12 @@ int posv_backward(hipStream_t s, int recompute, Mat<T> F, Mat<T> X, Mat<T> Xb, M
         TRY(dispatch_trsm<T>(s, 0, 0, 0, T(1), F, Xb, Bb, batch));
         TRY(dispatch_trsm<T>(s, 0, 1, 0, T(1), transposed(F), Bb, Bb, batch));
     }
-    LAUNCH_EW((sym_outer_lower_kernel<T>), batch, Bb, X, Ab);
+    if (batch > 0 && Ab.rows > 0) {
+        unsigned const groups = unsigned((Ab.rows + kFoldCols - 1) / kFoldCols);
+        hipLaunchKernelGGL((sym_outer_lower_kernel<T>), dim3(unsigned(batch), groups), dim3(kEw), 0, s, Bb, X, Ab);
+        int const r_ = check_launch("sym_outer_lower_kernel");
+        if (r_) return r_;
+    }
     return 0;
 }
 
