@@ -159,7 +159,9 @@ __device__ __forceinline__ void chol_owner_body(Mat<T> const& A, Mat<T> const& L
                 acc_to_lds(sD, dt, lane);
                 int const bk = diag_factor_inl<double>(sD, sW, lane);
                 if (lane == 0 && bk && sBad == 0 && 16 * kb + bk <= n) sBad = 16 * kb + bk;
-                owner_store<PANEL>(l, lpanel, n, kb, kb, lds_to_acc<double>(sD, lane), lane);
+                // (opaque indices: the store's address terms formed here, not
+                // hoisted out of the block loop into registers)
+                owner_store<PANEL>(l, lpanel, n, opaque(kb), opaque(kb), lds_to_acc<double>(sD, lane), lane);
                 d4 const wv = lds_to_acc<double>(sW, lane);
 #pragma unroll
                 for (int s = 0; s < S; ++s)
@@ -349,12 +351,12 @@ int launch_chol_owner(hipStream_t s, Mat<double> A, Mat<double> L, Mat<double> B
         case 8: return launch_owner_nt<8, 4, 2>(s, A, L, B, X, n, nrhs, batch, info, only_if);
         case 9: return launch_owner_nt<9, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
         case 10: return launch_owner_nt<10, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
-        case 11: return launch_owner_nt<11, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
-        case 12: return launch_owner_nt<12, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
-        case 13: return launch_owner_nt<13, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
-        case 14: return launch_owner_nt<14, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
-        case 15: return launch_owner_nt<15, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
-        default: return launch_owner_nt<16, 8>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 11: return launch_owner_nt<11, 16>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 12: return launch_owner_nt<12, 16>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 13: return launch_owner_nt<13, 16>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 14: return launch_owner_nt<14, 16>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        case 15: return launch_owner_nt<15, 16>(s, A, L, B, X, n, nrhs, batch, info, only_if);
+        default: return launch_owner_nt<16, 16>(s, A, L, B, X, n, nrhs, batch, info, only_if);
     }
 }
 
