@@ -386,12 +386,12 @@ int launch_chol_owner_f32(hipStream_t s, Mat<float> A, Mat<float> L, Mat<float> 
         case 8: return launch_owner_nt_f32<8, 4, 2>(s, A, L, B, X, n, nrhs, batch, info);
         case 9: return launch_owner_nt_f32<9, 8>(s, A, L, B, X, n, nrhs, batch, info);
         case 10: return launch_owner_nt_f32<10, 8>(s, A, L, B, X, n, nrhs, batch, info);
-        case 11: return launch_owner_nt_f32<11, 8>(s, A, L, B, X, n, nrhs, batch, info);
-        case 12: return launch_owner_nt_f32<12, 8>(s, A, L, B, X, n, nrhs, batch, info);
-        case 13: return launch_owner_nt_f32<13, 8>(s, A, L, B, X, n, nrhs, batch, info);
-        case 14: return launch_owner_nt_f32<14, 8>(s, A, L, B, X, n, nrhs, batch, info);
-        case 15: return launch_owner_nt_f32<15, 8>(s, A, L, B, X, n, nrhs, batch, info);
-        default: return launch_owner_nt_f32<16, 8>(s, A, L, B, X, n, nrhs, batch, info);
+        case 11: return launch_owner_nt_f32<11, 16>(s, A, L, B, X, n, nrhs, batch, info);
+        case 12: return launch_owner_nt_f32<12, 16>(s, A, L, B, X, n, nrhs, batch, info);
+        case 13: return launch_owner_nt_f32<13, 16>(s, A, L, B, X, n, nrhs, batch, info);
+        case 14: return launch_owner_nt_f32<14, 16>(s, A, L, B, X, n, nrhs, batch, info);
+        case 15: return launch_owner_nt_f32<15, 16>(s, A, L, B, X, n, nrhs, batch, info);
+        default: return launch_owner_nt_f32<16, 16>(s, A, L, B, X, n, nrhs, batch, info);
     }
 }
 
