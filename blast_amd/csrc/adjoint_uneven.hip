@@ -222,7 +222,7 @@ __device__ __forceinline__ void adj_instance(AdjLds& sm, Ref<double> const& L, R
     }
 }
 
-__global__ __launch_bounds__(kAdjT, 2) void posv_backward_uneven_kernel(const blast_adjoint_instance_t* inst,
+__global__ __launch_bounds__(kAdjT, 3) void posv_backward_uneven_kernel(const blast_adjoint_instance_t* inst,
                                                                         int64_t count, int32_t* queue, int32_t* info) {
     __shared__ AdjLds sm;
     __shared__ int sIdx;
