@@ -21,8 +21,9 @@
 // Scheduling: workgroups pull instance indices from queue[1..count] through
 // the counter queue[0] (atomic); the host orders the queue by size bucket,
 // largest first, so the tail of the launch is made of small instances.
-// The trailing-update strip width (1, 2 or 4 MFMA tiles per wavefront) is
-// taken per nx-bucket from the tuning table.
+// The trailing-update strip width (1 or 2 MFMA tiles per wavefront; a table
+// entry of 4 runs as 2) is taken per nx-bucket from the tuning table.  Three
+// workgroups per CU (at most 168 VGPRs).
 
 #include "dispatch.hpp"
 #include "launch.hpp"
@@ -108,8 +109,8 @@ __device__ void wg_gemm(double alpha, DV A, DV B, double beta, CV C, DV D, int M
 }
 
 // The same product with both operands staged through LDS: the workgroup
-// forms 64 x 16 tn blocks of D (wavefront w: rows 16 w..16 w + 15, tn = 1, 2
-// or 4 column tiles, a runtime choice so that one instantiation serves every
+// forms 64 x 16 tn blocks of D (wavefront w: rows 16 w..16 w + 15, tn = 1 or
+// 2 column tiles, a runtime choice so that one instantiation serves every
 // size bucket), K in chunks of 16 held in two LDS buffers (each element of a chunk
 // loaded once from memory and read by all four wavefronts, the next chunk's
 // loads in flight during the current chunk's MFMAs).  Loads follow each
@@ -121,7 +122,10 @@ template <int MODE, class CV = DV>
 __device__ void wg_gemm_lds(double alpha, DV A, DV B, double beta, CV C, DV D, int M, int N, int K, int tn,
                             double* sg) {
     using Mf = Mfma<double>;
-    constexpr int BM = 64, KC = kGemmKc, LS = kGemmLs, TN = 4;
+    // at most 2 tiles per strip: the 32 VGPRs of two more accumulators would
+    // cost the third workgroup per CU (14 % on the uneven 20000 x 10 batch)
+    constexpr int BM = 64, KC = kGemmKc, LS = kGemmLs, TN = 2;
+    tn = tn < TN ? tn : TN;
     int const lbn = tn == 4 ? 6 : (tn == 2 ? 5 : 4), BN = 1 << lbn;
     // (lane-dependent indices formed here: hoisted out of a caller's loop they
     // would stay live, and be saved, across its calls)
@@ -363,7 +367,7 @@ __device__ void riccati_instance(RiccatiInstance const& ins, int horizon, int tn
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kT, 2) void riccati_persistent_kernel(const RiccatiInstance* inst, int64_t start,
+__global__ __launch_bounds__(kT, 3) void riccati_persistent_kernel(const RiccatiInstance* inst, int64_t start,
                                                                 int64_t count, int horizon, const double* in,
                                                                 double* out, double* work, int32_t* queue,
                                                                 RiccatiTuning tuning, int32_t* info) {

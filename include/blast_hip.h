@@ -390,7 +390,8 @@ int blast_hip_driccati_batched(void* stream, int horizon, const blast_riccati_in
                                const double* in, double* out, double* work, int32_t* queue, int32_t* info);
 
 /* Strip-width tuning table (MFMA 16x16 tiles per wavefront in the trailing
- * updates, 1, 2 or 4) indexed by nx bucket min(nx/16, 15); process-wide. */
+ * updates, 1, 2 or 4; 4 runs as 2, the kernel's strips holding at most two
+ * tiles) indexed by nx bucket min(nx/16, 15); process-wide. */
 int blast_hip_riccati_set_tuning(const int8_t* strip_width16);
 int blast_hip_riccati_get_tuning(int8_t* strip_width16);
 

@@ -31,7 +31,8 @@ RULES = {
     "getrf_panel_kernel": dict(scratch=0, vgpr_spill=0, sgpr_spill=0, calls=False),
     "getrf_left_swaps_kernel": dict(scratch=0, vgpr_spill=0, sgpr_spill=0, calls=False),
     "sposv_ir_kernel": dict(scratch=0, vgpr_spill=0, calls=True),
-    "riccati_persistent_kernel": dict(scratch=12, vgpr_spill=0, calls=True),
+    # scratch: the call frame of the noinline triangular solve (no VGPR spill)
+    "riccati_persistent_kernel": dict(scratch=48, vgpr_spill=0, calls=True),
 }
 
 
